@@ -1,0 +1,130 @@
+/*
+ * vt_api.h -- C ABI of the MI355X (gfx950) implementation of Vortex-Transport's
+ * DG right-hand side and RK4 loop.
+ *
+ * Plain C types only.  Host arrays use the reference's element-major order:
+ *   vertices   [n][3][2]  (Element.H:27, Meshgeneration.H:9)
+ *   neighbors  [n][3]     right, left, vertical  (Meshgeneration.H:10, .cpp:246-252)
+ *   type       [n]        0 right angle down, 1 up (Meshgeneration.H:11)
+ *   U          [n][Np][4] rho, rho u, rho v, rho E (Element.H:31)
+ *   F          [n][Np][2][4]                       (Element.H:32)
+ * Np = (p+1)(p+2)/2.  Every function returns a vt_status (0 = VT_OK) unless
+ * noted.  A context is single-threaded and stream-ordered; separate contexts
+ * are independent.  Device memory is owned by the context; host arrays are
+ * owned by the caller and are only read or written during the call.
+ */
+#ifndef VT_API_H
+#define VT_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum vt_status {
+    VT_OK = 0,
+    VT_ERR_ARG = 1,            /* bad size, pointer or connectivity */
+    VT_ERR_UNSUPPORTED_P = 2,  /* p outside 0..3 (Lagrangebasis.cpp:66-69) */
+    VT_ERR_QUAD = 3,           /* integration order outside 0..19 (Quadraturerule.cpp:164-167) */
+    VT_ERR_NONPHYSICAL = 4,    /* rho <= 0, p <= 0 or c2 <= 0 in the Roe flux (Numericalflux.cpp:51-124) */
+    VT_ERR_HIP = 5,            /* HIP runtime error, see vt_last_hip_error() */
+    VT_ERR_INIT_TYPE = 6,      /* initialisation type not 0 or 1 (Element.cpp:327-330) */
+    VT_ERR_METHOD = 7,         /* stepping method not 0 or 1 (main.cpp:166-169) */
+    VT_ERR_MESH = 8            /* mesh sizes not supported (ny < 2) */
+} vt_status;
+
+typedef struct vt_ctx vt_ctx;
+
+/* Text for a status code (static storage). */
+const char *vt_status_string(int status);
+/* Last HIP error string recorded by the library (static storage). */
+const char *vt_last_hip_error(void);
+
+/* ---- set-up (host) ------------------------------------------------------ */
+
+/* Structured periodic mesh; replaces generate_mesh (Meshgeneration.H:15,
+ * Meshgeneration.cpp:16-182) without the grid/ dump.  Cell size and origin are
+ * computed in float as the reference does; perturb_grid enters through the
+ * reference's sin/cos(pow()) term (NaN coordinates result for half-widths
+ * above 6, as in the reference).  Arrays hold 2*nx*ny elements. */
+int vt_mesh_generate(int nx, int ny, float domain_x, float domain_y, float perturb_grid,
+                     double *vertices, int *neighbors, int *type);
+/* Same connectivity and grid with every interior vertex moved by
+ * amplitude * h * U(-1,1) per component (splitmix64 stream from seed).  No
+ * reference counterpart: stands in for generate_mesh where its perturbation
+ * cannot be used (domains wider than 12). */
+int vt_mesh_generate_seeded(int nx, int ny, float domain_x, float domain_y, double amplitude,
+                            uint64_t seed, double *vertices, int *neighbors, int *type);
+/* Reference nodes [Np][2]; replaces generate_nodes_reference_space (Meshgeneration.H:17). */
+int vt_nodes_reference(int p, double *nodes);
+/* Reference operators [Np][Np], [Np][Np], [2][Np][Np]; replaces
+ * inverse_mass_matrix_reference_space and sitffness_matrix_reference_space
+ * (Preevolve.H:10,19).  Any pointer may be NULL. */
+int vt_reference_operators(int p, int integration_order, double *M, double *Minv, double *S);
+/* Quadrature sizes and points; replaces gauss_line_integral / gauss_area_integral
+ * (Quadraturerule.H:7,10).  line_xw [nq][2], area_xyw [na][3]; NULL to query sizes. */
+int vt_quadrature(int integration_order, int *nq_line, double *line_xw, int *n_area, double *area_xyw);
+
+/* ---- solver context ----------------------------------------------------- */
+
+/* Creates a context on the current HIP device and uploads geometry and
+ * connectivity.  Replaces the element set-up loop of main.cpp:115-147: the
+ * Element constructor, build_jacobians, build_mass_matrix_inverse,
+ * build_stiffness_matrix (Element.cpp:21-116) and the Evolve_element
+ * constructor with evaluate_basis_in_quadrature_poits (Evolve.cpp:24-132). */
+int vt_create(int n_elem, const double *vertices, const int *neighbors, const int *type, int p,
+              int integration_order, vt_ctx **out);
+void vt_destroy(vt_ctx *ctx);
+int vt_num_elements(const vt_ctx *ctx);
+int vt_num_nodes(const vt_ctx *ctx);
+
+/* Initial state; replaces Element::initialize_hydrodinamics (Element.cpp:119-333).
+ * init_type 0 = nodal interpolation, 1 = least-squares projection.  The
+ * reference hard-codes epsilon = 0.3, x0 = y0 = 0 (Element.cpp:136,142-143). */
+int vt_init_vortex(vt_ctx *ctx, int init_type, double epsilon, double x0, double y0);
+/* Exact vortex at the nodes at time t (the init_type 0 formula with t > 0). */
+int vt_exact_vortex(const vt_ctx *ctx, double t, double epsilon, double x0, double y0, double *U);
+
+int vt_set_state(vt_ctx *ctx, const double *U);
+int vt_get_state(vt_ctx *ctx, double *U);
+/* Nodal flux of the current state, computed as Timestepping.cpp:59-76 refreshes
+ * Element::hidrodynamics_vector_F after every state update. */
+int vt_get_flux(vt_ctx *ctx, double *F);
+/* Simulation time (Element::time, advanced by dt each step as Timestepping.cpp:91,315). */
+double vt_get_time(const vt_ctx *ctx);
+int vt_set_time(vt_ctx *ctx, double t);
+
+/* dU/dt of the current state [n][Np][4]; replaces evolve_elem
+ * (Timestepping.cpp:18-36) and Evolve_element's six RHS methods
+ * (Evolve.cpp:135-307). */
+int vt_rhs(vt_ctx *ctx, double *dUdt);
+/* nsteps steps of forward Euler (method 0, Timestepping.cpp:39-82) or RK4
+ * (method 1, Timestepping.cpp:85-305); replaces the time loop body of
+ * main.cpp:154-170.  Steps are replayed from captured HIP graphs.  Returns
+ * VT_ERR_NONPHYSICAL at the first failing flux evaluation, which
+ * vt_last_error() locates; the state is then undefined (the reference exits). */
+int vt_step(vt_ctx *ctx, int method, double dt, int nsteps);
+/* Location of the last non-physical state: element, quadrature point, side
+ * (0..2), code (1 left, 2 right, 3 Roe c2, as Numericalflux.cpp:52,85,121),
+ * step (1-based within the vt_step call) and stage (1..4). */
+int vt_last_error(const vt_ctx *ctx, int *info /* [6] */);
+
+/* ---- measurement and checks ---------------------------------------------- */
+
+/* Same as vt_step, timed with HIP events recorded on the context's stream
+ * around the graph launches: *ms = elapsed milliseconds. */
+int vt_step_timed(vt_ctx *ctx, int method, double dt, int nsteps, double *ms);
+/* Launches the four RK4 stage kernels of nsteps steps one by one, each
+ * between its own pair of HIP events on the context's stream;
+ * ms_per_stage[s] = average duration of stage s+1. */
+int vt_stage_kernel_times(vt_ctx *ctx, double dt, int nsteps, double *ms_per_stage /* [4] */);
+/* Device Roe flux on n independent inputs (records of UL[4] UR[4] n[2]);
+ * replaces numerical_flux (Numericalflux.H:7) for checking.  codes[i] is 0 or
+ * the reference's error number. */
+int vt_numerical_flux(const double *records, double *flux /* [n][4] */, int *codes, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

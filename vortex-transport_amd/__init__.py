@@ -1,0 +1,354 @@
+"""MI355X (gfx950) implementation of Vortex-Transport's DG right-hand side and RK4 loop.
+
+Python host mirror of the reference's call surface over the C ABI in
+include/vt_api.h (libvt_hip.so, built in-tree by build.py):
+
+  read_input_files(path)        Parameters.cpp:12-64   -> Parameters
+  generate_mesh(params)         Meshgeneration.cpp:16-182 -> Mesh
+  generate_nodes_reference_space(params)  Meshgeneration.cpp:188-219
+  Solver(mesh, p, order)        the Element / Evolve_element arrays of main.cpp:115-147
+    .initialize_hydrodinamics() Element.cpp:119-333
+    .evolve_elem()              Timestepping.cpp:18-36 (returns dU/dt)
+    .rk4(dt) / .forward_euler(dt)  Timestepping.cpp:85-305 / 39-82
+    .step(method, dt, nsteps)   the time loop of main.cpp:154-170
+  numerical_flux(UL, UR, n)     Numericalflux.cpp:13-156 (device)
+
+Non-physical states raise NonPhysicalStateError where the reference prints
+"N. Error: Non-physical state!" and exits (Numericalflux.cpp:51-124).  The
+package never falls back to a CPU path: if the HIP library cannot be loaded,
+import of the solver fails.
+"""
+import ctypes
+import math
+import os
+import re
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvt_hip.so")
+
+VT_OK, VT_ERR_ARG, VT_ERR_UNSUPPORTED_P, VT_ERR_QUAD, VT_ERR_NONPHYSICAL = 0, 1, 2, 3, 4
+VT_ERR_HIP, VT_ERR_INIT_TYPE, VT_ERR_METHOD, VT_ERR_MESH = 5, 6, 7, 8
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+class VortexError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+class NonPhysicalStateError(VortexError):
+    """Raised where the reference exits on rho <= 0, p <= 0 or c2 <= 0."""
+
+    def __init__(self, info):
+        elem, q, side, code, step, stage = info
+        msg = "%d. Error: Non-physical state! (element %d, side %d, quadrature point %d, step %d, stage %d)" % (
+            code, elem, side + 1, q, step, stage)
+        super().__init__(VT_ERR_NONPHYSICAL, msg)
+        self.info = info
+
+
+def lib():
+    """Loads libvt_hip.so (fails loudly when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libvt_hip.so not built: run __graft_entry__.build() (or make -C vortex-transport_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        sig = {
+            "vt_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+            "vt_last_hip_error": (ctypes.c_char_p, []),
+            "vt_mesh_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _dp, _ip, _ip]),
+            "vt_mesh_generate_seeded": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_double,
+                                                       ctypes.c_uint64, _dp, _ip, _ip]),
+            "vt_nodes_reference": (ctypes.c_int, [ctypes.c_int, _dp]),
+            "vt_reference_operators": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
+            "vt_quadrature": (ctypes.c_int, [ctypes.c_int, _ip, _dp, _ip, _dp]),
+            "vt_create": (ctypes.c_int, [ctypes.c_int, _dp, _ip, _ip, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]),
+            "vt_destroy": (None, [vp]),
+            "vt_num_elements": (ctypes.c_int, [vp]),
+            "vt_num_nodes": (ctypes.c_int, [vp]),
+            "vt_init_vortex": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+            "vt_exact_vortex": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
+            "vt_set_state": (ctypes.c_int, [vp, _dp]),
+            "vt_get_state": (ctypes.c_int, [vp, _dp]),
+            "vt_get_flux": (ctypes.c_int, [vp, _dp]),
+            "vt_get_time": (ctypes.c_double, [vp]),
+            "vt_set_time": (ctypes.c_int, [vp, ctypes.c_double]),
+            "vt_rhs": (ctypes.c_int, [vp, _dp]),
+            "vt_step": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+            "vt_last_error": (ctypes.c_int, [vp, _ip]),
+            "vt_step_timed": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
+            "vt_stage_kernel_times": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, _dp]),
+            "vt_numerical_flux": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _check(rc, solver=None):
+    if rc == VT_OK:
+        return
+    if rc == VT_ERR_NONPHYSICAL and solver is not None:
+        info = np.zeros(6, dtype=np.int32)
+        lib().vt_last_error(solver._h, _i(info))
+        raise NonPhysicalStateError(tuple(int(x) for x in info))
+    msg = lib().vt_status_string(rc).decode()
+    if rc == VT_ERR_HIP:
+        msg += ": " + lib().vt_last_hip_error().decode()
+    raise VortexError(rc, msg)
+
+
+def num_nodes(p):
+    return (p + 1) * (p + 2) // 2
+
+
+# ---------------------------------------------------------------------------
+# Parameters (Parameters.H:5-19, Parameters.cpp:12-64)
+# ---------------------------------------------------------------------------
+
+_INT_RE = re.compile(r"\s*([+-]?\d+)")
+_FLT_RE = re.compile(r"\s*([+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?|[+-]?(?:inf(?:inity)?|nan))", re.I)
+
+
+def _stoi(s):
+    m = _INT_RE.match(s)
+    if not m:
+        raise ValueError("stoi: no conversion for %r" % s)
+    v = int(m.group(1))
+    if not -2**31 <= v < 2**31:
+        raise ValueError("stoi: out of range %r" % s)
+    return v
+
+
+def _stod(s):
+    m = _FLT_RE.match(s)
+    if not m:
+        raise ValueError("stod: no conversion for %r" % s)
+    return float(m.group(1))
+
+
+def _stof(s):
+    return float(np.float32(_stod(s)))
+
+
+@dataclass
+class Parameters:
+    """struct parameters (Parameters.H:5-19); float fields hold float32 values."""
+    p: int
+    domain_x: float
+    domain_y: float
+    num_element_in_x: int
+    num_element_in_y: int
+    perturb_grid: float
+    integration_order: int
+    simulation_time: float
+    number_time_steps: int
+    time_step: float
+    write_every_steps: int
+    U_initialization_type: int
+    stepping_method: int
+
+
+def read_input_files(path):
+    """Parses a key=value input file as Parameters.cpp:12-64 does: a line
+    without '=' is ignored, values keep trailing text that stoi/stof/stod
+    tolerate, perturb_grid goes through stoi, and
+    time_step = stod(simulation_time) / stod(number_time_steps)."""
+    kv = {}
+    with open(path) as f:
+        for line in f.read().split("\n"):
+            pos = line.find("=")
+            if pos >= 0:
+                kv[line[:pos]] = line[pos + 1:]
+    g = lambda k: kv.get(k, "")
+    return Parameters(
+        p=_stoi(g("p")), domain_x=_stof(g("domain_x")), domain_y=_stof(g("domain_y")),
+        num_element_in_x=_stoi(g("num_element_in_x")), num_element_in_y=_stoi(g("num_element_in_y")),
+        perturb_grid=float(_stoi(g("perturb_grid"))), integration_order=_stoi(g("integration_order")),
+        simulation_time=_stof(g("simulation_time")), number_time_steps=_stoi(g("number_time_steps")),
+        time_step=_stod(g("simulation_time")) / _stod(g("number_time_steps")),
+        write_every_steps=_stoi(g("write_every_steps")), U_initialization_type=_stoi(g("U_initialization_type")),
+        stepping_method=_stoi(g("stepping_method")))
+
+
+# ---------------------------------------------------------------------------
+# Mesh (Meshgeneration.H:7-17)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Mesh:
+    """struct mesh: vertices [n][3][2], elements_at_boundary [n][3] (right,
+    left, vertical), element_type [n]."""
+    element_coordinates: np.ndarray
+    elements_at_boundary: np.ndarray
+    element_type: np.ndarray
+
+    @property
+    def num_elements(self):
+        return self.element_coordinates.shape[0]
+
+
+def generate_mesh(params=None, nx=None, ny=None, domain_x=10.0, domain_y=10.0, perturb_grid=0.0):
+    """Structured periodic mesh with the reference connectivity (Meshgeneration.cpp:16-182)."""
+    if params is not None:
+        nx, ny = params.num_element_in_x, params.num_element_in_y
+        domain_x, domain_y, perturb_grid = params.domain_x, params.domain_y, params.perturb_grid
+    ne = 2 * nx * ny
+    v = np.zeros((ne, 3, 2))
+    b = np.zeros((ne, 3), dtype=np.int32)
+    t = np.zeros(ne, dtype=np.int32)
+    _check(lib().vt_mesh_generate(nx, ny, domain_x, domain_y, perturb_grid, _d(v), _i(b), _i(t)))
+    return Mesh(v, b, t)
+
+
+def generate_mesh_seeded(nx, ny, domain_x, domain_y, amplitude, seed):
+    """Reference connectivity with seeded uniform interior-vertex offsets of amplitude*h."""
+    ne = 2 * nx * ny
+    v = np.zeros((ne, 3, 2))
+    b = np.zeros((ne, 3), dtype=np.int32)
+    t = np.zeros(ne, dtype=np.int32)
+    _check(lib().vt_mesh_generate_seeded(nx, ny, domain_x, domain_y, amplitude, seed, _d(v), _i(b), _i(t)))
+    return Mesh(v, b, t)
+
+
+def generate_nodes_reference_space(p):
+    if not isinstance(p, int):
+        p = p.p
+    x = np.zeros((num_nodes(p), 2))
+    _check(lib().vt_nodes_reference(p, _d(x)))
+    return x
+
+
+def reference_operators(p, order):
+    """(M_ref, M_ref^-1, S_ref[2]) as Preevolve.cpp:15-144 computes them."""
+    n = num_nodes(p)
+    M, Mi, S = np.zeros((n, n)), np.zeros((n, n)), np.zeros((2, n, n))
+    _check(lib().vt_reference_operators(p, order, _d(M), _d(Mi), _d(S)))
+    return M, Mi, S
+
+
+def quadrature(order):
+    nl, na = ctypes.c_int(), ctypes.c_int()
+    _check(lib().vt_quadrature(order, ctypes.byref(nl), None, ctypes.byref(na), None))
+    xw, xyw = np.zeros((nl.value, 2)), np.zeros((na.value, 3))
+    _check(lib().vt_quadrature(order, None, _d(xw), None, _d(xyw)))
+    return xw, xyw
+
+
+def numerical_flux(UL, UR, n):
+    """Device Roe flux on arrays UL[...,4], UR[...,4], n[...,2]; returns (F, codes)."""
+    UL, UR, n = np.asarray(UL, float), np.asarray(UR, float), np.asarray(n, float)
+    shape = UL.shape[:-1]
+    cnt = int(np.prod(shape)) if shape else 1
+    rec = np.ascontiguousarray(np.concatenate([UL.reshape(cnt, 4), UR.reshape(cnt, 4), n.reshape(cnt, 2)], axis=1))
+    F = np.zeros((cnt, 4))
+    codes = np.zeros(cnt, dtype=np.int32)
+    _check(lib().vt_numerical_flux(_d(rec), _d(F), _i(codes), cnt))
+    return F.reshape(shape + (4,)), codes.reshape(shape)
+
+
+# ---------------------------------------------------------------------------
+# Solver (the Element / Evolve_element arrays of main.cpp:115-147)
+# ---------------------------------------------------------------------------
+
+class Solver:
+    """Device-resident DG solver for one mesh.
+
+    The state stays in HBM between calls; state()/flux() copy it out in the
+    reference's [n][Np][4] / [n][Np][2][4] order."""
+
+    def __init__(self, mesh, p, integration_order):
+        self.mesh = mesh
+        self.p = p
+        self.order = integration_order
+        self.np = num_nodes(p)
+        self.ne = mesh.num_elements
+        self._v = np.ascontiguousarray(mesh.element_coordinates, dtype=np.float64)
+        self._b = np.ascontiguousarray(mesh.elements_at_boundary, dtype=np.int32)
+        self._t = np.ascontiguousarray(mesh.element_type, dtype=np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().vt_create(self.ne, _d(self._v), _i(self._b), _i(self._t), p, integration_order, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.vt_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def initialize_hydrodinamics(self, init_type=0, epsilon=0.3, x0=0.0, y0=0.0):
+        _check(lib().vt_init_vortex(self._h, init_type, epsilon, x0, y0), self)
+
+    def exact_vortex(self, t, epsilon=0.3, x0=0.0, y0=0.0):
+        U = np.zeros((self.ne, self.np, 4))
+        _check(lib().vt_exact_vortex(self._h, t, epsilon, x0, y0, _d(U)))
+        return U
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.ne, self.np, 4)
+        _check(lib().vt_set_state(self._h, _d(U)))
+
+    def state(self):
+        U = np.zeros((self.ne, self.np, 4))
+        _check(lib().vt_get_state(self._h, _d(U)))
+        return U
+
+    def flux(self):
+        F = np.zeros((self.ne, self.np, 2, 4))
+        _check(lib().vt_get_flux(self._h, _d(F)))
+        return F
+
+    @property
+    def time(self):
+        return lib().vt_get_time(self._h)
+
+    @time.setter
+    def time(self, t):
+        _check(lib().vt_set_time(self._h, t))
+
+    def evolve_elem(self):
+        d = np.zeros((self.ne, self.np, 4))
+        _check(lib().vt_rhs(self._h, _d(d)), self)
+        return d
+
+    def step(self, method, dt, nsteps=1):
+        _check(lib().vt_step(self._h, method, dt, nsteps), self)
+
+    def rk4(self, dt, nsteps=1):
+        self.step(1, dt, nsteps)
+
+    def forward_euler(self, dt, nsteps=1):
+        self.step(0, dt, nsteps)
+
+    def step_timed(self, method, dt, nsteps):
+        ms = ctypes.c_double()
+        _check(lib().vt_step_timed(self._h, method, dt, nsteps, ctypes.byref(ms)), self)
+        return ms.value
+
+    def stage_kernel_times(self, dt, nsteps):
+        ms = np.zeros(4)
+        _check(lib().vt_stage_kernel_times(self._h, dt, nsteps, _d(ms)), self)
+        return ms

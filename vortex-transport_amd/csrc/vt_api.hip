@@ -1,0 +1,481 @@
+// vt_api.hip -- C ABI over the gfx950 kernels: device buffers, set-up upload,
+// RK4 / forward-Euler stepping through captured HIP graphs, error reporting.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "vt_api.h"
+#include "vt_device.hpp"
+#include "vt_kernels.hpp"
+#include "vt_setup.hpp"
+
+namespace {
+
+thread_local char g_hip_error[256] = "";
+
+bool hip_ok(hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    std::snprintf(g_hip_error, sizeof(g_hip_error), "%s: %s", what, hipGetErrorString(e));
+    return false;
+}
+#define HIP_TRY(expr)                                   \
+    do {                                                \
+        if (!hip_ok((expr), #expr)) return VT_ERR_HIP;  \
+    } while (0)
+
+// Plane stride: element count rounded to 256 plus a 64-element skew, so the
+// 4*Np planes of one element do not all start at the same HBM channel offset.
+int plane_stride(int ne) { return ((ne + 255) / 256) * 256 + 64; }
+
+// Graph steps per replay: one graph covers up to kGraphSteps steps.
+constexpr int kGraphSteps = 64;
+
+}  // namespace
+
+struct vt_ctx {
+    int ne = 0, ld = 0, p = 0, np = 0, nq = 0, order = 0, device = 0;
+    hipStream_t stream = nullptr;
+    double *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr, *vtx = nullptr, *tab = nullptr;
+    int* nbr = nullptr;
+    unsigned long long* err = nullptr;       // device error key
+    unsigned long long* err_init = nullptr;  // pinned host, holds kNoError
+    unsigned long long* err_host = nullptr;  // pinned host, one slot per replay
+    size_t err_host_cap = 0;
+    std::vector<double> vertices;
+    double time = 0.0;
+    std::map<std::tuple<int, unsigned long long, int>, hipGraphExec_t> graphs;
+    int last_err[6] = {-1, -1, -1, 0, 0, 0};
+};
+
+namespace {
+
+vt::StageParams base_params(vt_ctx* c) {
+    vt::StageParams prm{};
+    prm.vtx = c->vtx;
+    prm.nbr = c->nbr;
+    prm.tab = c->tab;
+    prm.err = c->err;
+    prm.ne = c->ne;
+    prm.ld = c->ld;
+    prm.blocks_per_xcd = vt::blocks_per_xcd(c->ne);
+    return prm;
+}
+
+// Launches the kernels of one RK4 step (4 stages) or one Euler step on the
+// context stream.  Un holds the state before and after the step, except for
+// an Euler step, which writes Un -> dst.
+hipError_t enqueue_rk4_step(vt_ctx* c, double dt, int stage_base) {
+    vt::StageParams prm = base_params(c);
+    prm.dt = dt;
+    struct S { const double* in; double* out; int mode; };
+    const S st[4] = {{c->Un, c->Ua, vt::kModeRk1}, {c->Ua, c->Ub, vt::kModeRk2},
+                     {c->Ub, c->Ua, vt::kModeRk3}, {c->Ua, c->Un, vt::kModeRk4}};
+    for (int s = 0; s < 4; ++s) {
+        prm.Uin = st[s].in;
+        prm.Uout = st[s].out;
+        prm.Un = (s == 1 || s == 2) ? c->Un : nullptr;
+        prm.acc = c->acc;
+        prm.mode = st[s].mode;
+        prm.stage_id = stage_base + s;
+        hipError_t e = vt::launch_stage(prm, c->p, c->nq, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t enqueue_euler_step(vt_ctx* c, double dt, const double* src, double* dst, int stage_id) {
+    vt::StageParams prm = base_params(c);
+    prm.dt = dt;
+    prm.Uin = src;
+    prm.Uout = dst;
+    prm.mode = vt::kModeEuler;
+    prm.stage_id = stage_id;
+    return vt::launch_stage(prm, c->p, c->nq, c->stream);
+}
+
+// Graph of k steps (Euler graphs hold an even number of steps, alternating
+// Un -> Ua -> Un).
+int get_graph(vt_ctx* c, int method, double dt, int k, hipGraphExec_t* out) {
+    unsigned long long dbits;
+    std::memcpy(&dbits, &dt, sizeof(dbits));
+    auto key = std::make_tuple(method, dbits, k);
+    auto it = c->graphs.find(key);
+    if (it != c->graphs.end()) { *out = it->second; return VT_OK; }
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < k && e == hipSuccess; ++i) {
+        if (method == 1) e = enqueue_rk4_step(c, dt, 4 * i);
+        else e = enqueue_euler_step(c, dt, (i % 2) ? c->Ua : c->Un, (i % 2) ? c->Un : c->Ua, i);
+    }
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (!hip_ok(e, "graph capture launch") || !hip_ok(e2, "hipStreamEndCapture")) return VT_ERR_HIP;
+    hipGraphExec_t ex;
+    hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (!hip_ok(e3, "hipGraphInstantiate")) return VT_ERR_HIP;
+    c->graphs[key] = ex;
+    *out = ex;
+    return VT_OK;
+}
+
+void to_planes(const vt_ctx* c, const double* U, std::vector<double>* planes) {
+    planes->assign(static_cast<size_t>(4) * c->np * c->ld, 0.0);
+    for (int e = 0; e < c->ne; ++e)
+        for (int j = 0; j < c->np; ++j)
+            for (int k = 0; k < 4; ++k)
+                (*planes)[static_cast<size_t>(k * c->np + j) * c->ld + e] = U[(static_cast<size_t>(e) * c->np + j) * 4 + k];
+}
+
+void from_planes(const vt_ctx* c, const std::vector<double>& planes, double* U) {
+    for (int e = 0; e < c->ne; ++e)
+        for (int j = 0; j < c->np; ++j)
+            for (int k = 0; k < 4; ++k)
+                U[(static_cast<size_t>(e) * c->np + j) * 4 + k] = planes[static_cast<size_t>(k * c->np + j) * c->ld + e];
+}
+
+int read_planes(vt_ctx* c, const double* dev, double* U) {
+    std::vector<double> h(static_cast<size_t>(4) * c->np * c->ld);
+    HIP_TRY(hipMemcpyAsync(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    from_planes(c, h, U);
+    return VT_OK;
+}
+
+void decode_error(vt_ctx* c, unsigned long long key, int step_base, int per_step) {
+    const unsigned long long stage = key >> 40;
+    unsigned long long t = key & ((1ull << 40) - 1);
+    c->last_err[3] = static_cast<int>(t % 4); t /= 4;
+    c->last_err[2] = static_cast<int>(t % 4); t /= 4;
+    c->last_err[1] = static_cast<int>(t % 16); t /= 16;
+    c->last_err[0] = static_cast<int>(t);
+    c->last_err[4] = step_base + static_cast<int>(stage / per_step) + 1;
+    c->last_err[5] = static_cast<int>(stage % per_step) + 1;
+}
+
+int ensure_err_slots(vt_ctx* c, size_t n) {
+    if (n <= c->err_host_cap) return VT_OK;
+    if (c->err_host) hipHostFree(c->err_host);
+    c->err_host = nullptr;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->err_host), n * sizeof(unsigned long long), hipHostMallocDefault));
+    c->err_host_cap = n;
+    return VT_OK;
+}
+
+int run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0, hipEvent_t ev1) {
+    if (method != 0 && method != 1) return VT_ERR_METHOD;
+    if (nsteps < 0) return VT_ERR_ARG;
+    std::vector<int> reps;
+    int left = nsteps;
+    while (left > 0) {
+        int k = left < kGraphSteps ? left : kGraphSteps;
+        if (method == 0 && k > 1 && (k % 2)) k -= 1;
+        reps.push_back(k);
+        left -= k;
+    }
+    int rc = ensure_err_slots(c, reps.size() + 1);
+    if (rc) return rc;
+    std::vector<hipGraphExec_t> ex(reps.size(), nullptr);
+    for (size_t r = 0; r < reps.size(); ++r)
+        if (!(method == 0 && reps[r] == 1)) {
+            rc = get_graph(c, method, dt, reps[r], &ex[r]);
+            if (rc) return rc;
+        }
+    if (ev0) HIP_TRY(hipEventRecord(ev0, c->stream));
+    for (size_t r = 0; r < reps.size(); ++r) {
+        HIP_TRY(hipMemcpyAsync(c->err, c->err_init, sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+        if (ex[r]) {
+            HIP_TRY(hipGraphLaunch(ex[r], c->stream));
+        } else {  // single Euler step: Un -> Ua, then copy back
+            HIP_TRY(enqueue_euler_step(c, dt, c->Un, c->Ua, 0));
+            HIP_TRY(hipMemcpyAsync(c->Un, c->Ua, sizeof(double) * 4 * c->np * c->ld, hipMemcpyDeviceToDevice, c->stream));
+        }
+        HIP_TRY(hipMemcpyAsync(c->err_host + r, c->err, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (ev1) HIP_TRY(hipEventRecord(ev1, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    int base = 0;
+    for (size_t r = 0; r < reps.size(); ++r) {
+        // Element::time advances by dt once per step (Timestepping.cpp:91,315).
+        if (c->err_host[r] != vt::kNoError) {
+            decode_error(c, c->err_host[r], base, method == 1 ? 4 : 1);
+            return VT_ERR_NONPHYSICAL;
+        }
+        for (int i = 0; i < reps[r]; ++i) c->time += dt;
+        base += reps[r];
+    }
+    return VT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vt_status_string(int s) {
+    switch (s) {
+    case VT_OK: return "ok";
+    case VT_ERR_ARG: return "invalid argument";
+    case VT_ERR_UNSUPPORTED_P: return "unsupported p order";
+    case VT_ERR_QUAD: return "unsupported quadrature order";
+    case VT_ERR_NONPHYSICAL: return "Non-physical state";
+    case VT_ERR_HIP: return "HIP error";
+    case VT_ERR_INIT_TYPE: return "unsupported initialization type";
+    case VT_ERR_METHOD: return "unsupported time stepping method";
+    case VT_ERR_MESH: return "unsupported mesh size";
+    default: return "unknown status";
+    }
+}
+
+const char* vt_last_hip_error(void) { return g_hip_error; }
+
+int vt_mesh_generate(int nx, int ny, float dx, float dy, float perturb, double* v, int* nb, int* t) {
+    if (!v || !nb || !t) return VT_ERR_ARG;
+    return vt::generate_mesh(nx, ny, dx, dy, perturb, v, nb, t) ? VT_ERR_MESH : VT_OK;
+}
+
+int vt_mesh_generate_seeded(int nx, int ny, float dx, float dy, double amp, uint64_t seed, double* v, int* nb, int* t) {
+    if (!v || !nb || !t) return VT_ERR_ARG;
+    return vt::generate_mesh_seeded(nx, ny, dx, dy, amp, seed, v, nb, t) ? VT_ERR_MESH : VT_OK;
+}
+
+int vt_nodes_reference(int p, double* nodes) {
+    if (!nodes) return VT_ERR_ARG;
+    return vt::reference_nodes(p, nodes) ? VT_OK : VT_ERR_UNSUPPORTED_P;
+}
+
+int vt_reference_operators(int p, int order, double* M, double* Minv, double* S) {
+    if (p < 0 || p > vt::kMaxP) return VT_ERR_UNSUPPORTED_P;
+    if (vt::area_rule(order, nullptr) < 0) return VT_ERR_QUAD;
+    return vt::reference_operators(p, order, M, Minv, S) ? VT_OK : VT_ERR_ARG;
+}
+
+int vt_quadrature(int order, int* nq_line, double* line_xw, int* n_area, double* area_xyw) {
+    std::vector<double> a, b;
+    const int nl = vt::line_rule(order, &a), na = vt::area_rule(order, &b);
+    if (nl < 0 || na < 0) return VT_ERR_QUAD;
+    if (nq_line) *nq_line = nl;
+    if (n_area) *n_area = na;
+    if (line_xw) std::memcpy(line_xw, a.data(), sizeof(double) * a.size());
+    if (area_xyw) std::memcpy(area_xyw, b.data(), sizeof(double) * b.size());
+    return VT_OK;
+}
+
+int vt_create(int n, const double* vertices, const int* neighbors, const int* type, int p, int order, vt_ctx** out) {
+    if (!out) return VT_ERR_ARG;
+    *out = nullptr;
+    if (n < 1 || !vertices || !neighbors || !type) return VT_ERR_ARG;
+    if (p < 0 || p > vt::kMaxP) return VT_ERR_UNSUPPORTED_P;
+    vt::OperatorTable tab;
+    if (vt::line_rule(order, nullptr) < 0 || !vt::build_operator_table(p, order, &tab)) return VT_ERR_QUAD;
+    for (int e = 0; e < n; ++e) {
+        if (type[e] != 0 && type[e] != 1) return VT_ERR_ARG;
+        for (int s = 0; s < 3; ++s)
+            if (neighbors[3 * e + s] < 0 || neighbors[3 * e + s] >= n) return VT_ERR_ARG;
+    }
+    if (static_cast<long long>(plane_stride(n)) * 8 * 4 * vt::num_nodes(p) >= (1ll << 32) ||
+        static_cast<long long>(n) * 256 >= (1ll << 40))
+        return VT_ERR_ARG;  // 32-bit byte offsets and the 40-bit error key
+    vt_ctx* c = new vt_ctx();
+    c->ne = n; c->ld = plane_stride(n); c->p = p; c->np = vt::num_nodes(p); c->nq = tab.nq; c->order = order;
+    c->vertices.assign(vertices, vertices + 6 * static_cast<size_t>(n));
+    auto fail = [&](int rc) { vt_destroy(c); return rc; };
+    if (!hip_ok(hipGetDevice(&c->device), "hipGetDevice")) return fail(VT_ERR_HIP);
+    if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) return fail(VT_ERR_HIP);
+    const size_t sbytes = sizeof(double) * 4 * c->np * c->ld;
+    for (double** b : {&c->Un, &c->Ua, &c->Ub, &c->acc})
+        if (!hip_ok(hipMalloc(reinterpret_cast<void**>(b), sbytes), "hipMalloc state")) return fail(VT_ERR_HIP);
+    if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&c->vtx), sizeof(double) * 6 * c->ld), "hipMalloc vtx") ||
+        !hip_ok(hipMalloc(reinterpret_cast<void**>(&c->nbr), sizeof(int) * 3 * c->ld), "hipMalloc nbr") ||
+        !hip_ok(hipMalloc(reinterpret_cast<void**>(&c->tab), sizeof(double) * tab.data.size()), "hipMalloc tab") ||
+        !hip_ok(hipMalloc(reinterpret_cast<void**>(&c->err), sizeof(unsigned long long)), "hipMalloc err") ||
+        !hip_ok(hipHostMalloc(reinterpret_cast<void**>(&c->err_init), sizeof(unsigned long long), hipHostMallocDefault), "hipHostMalloc"))
+        return fail(VT_ERR_HIP);
+    *c->err_init = vt::kNoError;
+    std::vector<double> hv(static_cast<size_t>(6) * c->ld, 0.0);
+    std::vector<int> ns(static_cast<size_t>(3) * n), hn(static_cast<size_t>(3) * c->ld, 0);
+    vt::side_neighbours(n, neighbors, type, ns.data());
+    for (int e = 0; e < n; ++e) {
+        for (int k = 0; k < 6; ++k) hv[static_cast<size_t>(k) * c->ld + e] = vertices[6 * static_cast<size_t>(e) + k];
+        for (int s = 0; s < 3; ++s) hn[static_cast<size_t>(s) * c->ld + e] = ns[3 * static_cast<size_t>(e) + s];
+    }
+    if (!hip_ok(hipMemcpy(c->vtx, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice), "upload vtx") ||
+        !hip_ok(hipMemcpy(c->nbr, hn.data(), sizeof(int) * hn.size(), hipMemcpyHostToDevice), "upload nbr") ||
+        !hip_ok(hipMemcpy(c->tab, tab.data.data(), sizeof(double) * tab.data.size(), hipMemcpyHostToDevice), "upload tab") ||
+        !hip_ok(hipMemset(c->Un, 0, sbytes), "memset") || !hip_ok(hipMemset(c->Ua, 0, sbytes), "memset") ||
+        !hip_ok(hipMemset(c->Ub, 0, sbytes), "memset") || !hip_ok(hipMemset(c->acc, 0, sbytes), "memset"))
+        return fail(VT_ERR_HIP);
+    *out = c;
+    return VT_OK;
+}
+
+void vt_destroy(vt_ctx* c) {
+    if (!c) return;
+    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+    for (void* b : {static_cast<void*>(c->Un), static_cast<void*>(c->Ua), static_cast<void*>(c->Ub),
+                    static_cast<void*>(c->acc), static_cast<void*>(c->vtx), static_cast<void*>(c->nbr),
+                    static_cast<void*>(c->tab), static_cast<void*>(c->err)})
+        if (b) hipFree(b);
+    if (c->err_init) hipHostFree(c->err_init);
+    if (c->err_host) hipHostFree(c->err_host);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int vt_num_elements(const vt_ctx* c) { return c ? c->ne : 0; }
+int vt_num_nodes(const vt_ctx* c) { return c ? c->np : 0; }
+
+int vt_init_vortex(vt_ctx* c, int init_type, double epsilon, double x0, double y0) {
+    if (!c) return VT_ERR_ARG;
+    std::vector<double> U(static_cast<size_t>(c->ne) * c->np * 4);
+    if (!vt::initial_state(c->ne, c->vertices.data(), c->p, c->order, init_type, epsilon, x0, y0, c->time, U.data()))
+        return VT_ERR_INIT_TYPE;
+    return vt_set_state(c, U.data());
+}
+
+int vt_exact_vortex(const vt_ctx* c, double t, double epsilon, double x0, double y0, double* U) {
+    if (!c || !U) return VT_ERR_ARG;
+    vt::initial_state(c->ne, c->vertices.data(), c->p, c->order, 0, epsilon, x0, y0, t, U);
+    return VT_OK;
+}
+
+int vt_set_state(vt_ctx* c, const double* U) {
+    if (!c || !U) return VT_ERR_ARG;
+    std::vector<double> h;
+    to_planes(c, U, &h);
+    HIP_TRY(hipMemcpyAsync(c->Un, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
+int vt_get_state(vt_ctx* c, double* U) {
+    if (!c || !U) return VT_ERR_ARG;
+    return read_planes(c, c->Un, U);
+}
+
+int vt_get_flux(vt_ctx* c, double* F) {
+    if (!c || !F) return VT_ERR_ARG;
+    std::vector<double> U(static_cast<size_t>(c->ne) * c->np * 4);
+    int rc = read_planes(c, c->Un, U.data());
+    if (rc) return rc;
+    for (size_t i = 0; i < static_cast<size_t>(c->ne) * c->np; ++i) vt::nodal_flux(&U[4 * i], &F[8 * i]);
+    return VT_OK;
+}
+
+double vt_get_time(const vt_ctx* c) { return c ? c->time : 0.0; }
+int vt_set_time(vt_ctx* c, double t) {
+    if (!c) return VT_ERR_ARG;
+    c->time = t;
+    return VT_OK;
+}
+
+int vt_rhs(vt_ctx* c, double* dUdt) {
+    if (!c || !dUdt) return VT_ERR_ARG;
+    vt::StageParams prm = base_params(c);
+    prm.Uin = c->Un;
+    prm.Uout = c->Ub;
+    prm.mode = vt::kModeRhs;
+    prm.stage_id = 0;
+    HIP_TRY(hipMemcpyAsync(c->err, c->err_init, sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(vt::launch_stage(prm, c->p, c->nq, c->stream));
+    int rc = read_planes(c, c->Ub, dUdt);
+    if (rc) return rc;
+    unsigned long long key = 0;
+    HIP_TRY(hipMemcpy(&key, c->err, sizeof(key), hipMemcpyDeviceToHost));
+    if (key != vt::kNoError) {
+        decode_error(c, key, -1, 1);
+        return VT_ERR_NONPHYSICAL;
+    }
+    return VT_OK;
+}
+
+int vt_step(vt_ctx* c, int method, double dt, int nsteps) {
+    if (!c) return VT_ERR_ARG;
+    return run_steps(c, method, dt, nsteps, nullptr, nullptr);
+}
+
+int vt_step_timed(vt_ctx* c, int method, double dt, int nsteps, double* ms) {
+    if (!c || !ms) return VT_ERR_ARG;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = run_steps(c, method, dt, nsteps, e0, e1);
+    float f = 0.f;
+    if (rc == VT_OK && !hip_ok(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime")) rc = VT_ERR_HIP;
+    *ms = f;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+int vt_stage_kernel_times(vt_ctx* c, double dt, int nsteps, double* ms) {
+    if (!c || !ms || nsteps < 1) return VT_ERR_ARG;
+    std::vector<hipEvent_t> ev(8 * static_cast<size_t>(nsteps));
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    vt::StageParams prm = base_params(c);
+    prm.dt = dt;
+    const double* in[4] = {c->Un, c->Ua, c->Ub, c->Ua};
+    double* outp[4] = {c->Ua, c->Ub, c->Ua, c->Un};
+    HIP_TRY(hipMemcpyAsync(c->err, c->err_init, sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+    for (int i = 0; i < nsteps; ++i)
+        for (int s = 0; s < 4; ++s) {
+            prm.Uin = in[s];
+            prm.Uout = outp[s];
+            prm.Un = (s == 1 || s == 2) ? c->Un : nullptr;
+            prm.acc = c->acc;
+            prm.mode = vt::kModeRk1 + s;
+            prm.stage_id = 4 * i + s;
+            HIP_TRY(hipEventRecord(ev[8 * i + 2 * s], c->stream));
+            HIP_TRY(vt::launch_stage(prm, c->p, c->nq, c->stream));
+            HIP_TRY(hipEventRecord(ev[8 * i + 2 * s + 1], c->stream));
+        }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double acc[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nsteps; ++i)
+        for (int s = 0; s < 4; ++s) {
+            float f = 0.f;
+            HIP_TRY(hipEventElapsedTime(&f, ev[8 * i + 2 * s], ev[8 * i + 2 * s + 1]));
+            acc[s] += f;
+        }
+    for (auto& e : ev) hipEventDestroy(e);
+    for (int s = 0; s < 4; ++s) ms[s] = acc[s] / nsteps;
+    for (int i = 0; i < nsteps; ++i) c->time += dt;
+    unsigned long long key = 0;
+    HIP_TRY(hipMemcpy(&key, c->err, sizeof(key), hipMemcpyDeviceToHost));
+    if (key != vt::kNoError) {
+        decode_error(c, key, 0, 4);
+        return VT_ERR_NONPHYSICAL;
+    }
+    return VT_OK;
+}
+
+int vt_last_error(const vt_ctx* c, int* info) {
+    if (!c || !info) return VT_ERR_ARG;
+    std::memcpy(info, c->last_err, sizeof(c->last_err));
+    return VT_OK;
+}
+
+int vt_numerical_flux(const double* rec, double* flux, int* codes, int n) {
+    if (n < 0 || (n > 0 && (!rec || !flux || !codes))) return VT_ERR_ARG;
+    if (n == 0) return VT_OK;
+    double *din = nullptr, *dout = nullptr;
+    int* dc = nullptr;
+    int rc = VT_OK;
+    if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&din), sizeof(double) * 10 * n), "hipMalloc") ||
+        !hip_ok(hipMalloc(reinterpret_cast<void**>(&dout), sizeof(double) * 4 * n), "hipMalloc") ||
+        !hip_ok(hipMalloc(reinterpret_cast<void**>(&dc), sizeof(int) * n), "hipMalloc") ||
+        !hip_ok(hipMemcpy(din, rec, sizeof(double) * 10 * n, hipMemcpyHostToDevice), "upload") ||
+        !hip_ok(vt::launch_flux(din, dout, dc, n, nullptr), "flux kernel") ||
+        !hip_ok(hipMemcpy(flux, dout, sizeof(double) * 4 * n, hipMemcpyDeviceToHost), "download") ||
+        !hip_ok(hipMemcpy(codes, dc, sizeof(int) * n, hipMemcpyDeviceToHost), "download"))
+        rc = VT_ERR_HIP;
+    if (din) hipFree(din);
+    if (dout) hipFree(dout);
+    if (dc) hipFree(dc);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,318 @@
+// vt_kernels.hip -- gfx950 kernels for the DG right-hand side and the fused
+// RK4 stages (FP64).
+//
+// One thread per element.  State lives in structure-of-arrays planes
+// U[(k*NP + j)*ld + e] (k = conserved variable, j = node, e = element), so
+// each plane load/store of a wavefront is one contiguous 512-byte segment.
+// Per element and stage the kernel
+//   * loads its NP*4 state values and the P+1 edge-node values of each of its
+//     three neighbours (the only nodes whose basis functions are non-zero on
+//     the shared side),
+//   * forms the volume term  V = Sxi G^xi + Seta G^eta  from the nodal flux
+//     in contravariant form (G^xi = (y2-y0) Fx + (x0-x2) Fy, ...), which is
+//     S^x Fx + S^y Fy of Element.cpp:112-113 / Evolve.cpp:254-272 regrouped,
+//   * evaluates the Roe flux at the NQ Gauss points of each side
+//     (Numericalflux.cpp:22-166, including its ci = c2^2 and entropy fix) and
+//     subtracts the lifted flux (Evolve.cpp:225-251),
+//   * applies Minv_ref / detJ (Element.cpp:101, Evolve.cpp:290-307) and
+//   * performs the RK4 stage update (Timestepping.cpp:85-305) in registers.
+// Reference operators and edge tables are uniform across the grid and read
+// with scalar loads from one packed table.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "vt_device.hpp"
+#include "vt_kernels.hpp"
+
+#ifndef VT_Q_UNROLL
+#define VT_Q_UNROLL 1
+#endif
+#ifndef VT_J_UNROLL
+#define VT_J_UNROLL 1
+#endif
+#ifndef VT_MIN_WAVES
+#define VT_MIN_WAVES 2
+#endif
+
+namespace vt {
+
+template <int P>
+__device__ __forceinline__ constexpr int edge_node(int s, int m) {
+    if constexpr (P == 0) return 0;
+    else if constexpr (P == 1) return s == 0 ? (m == 0 ? 0 : 1) : s == 1 ? (m == 0 ? 1 : 2) : (m == 0 ? 2 : 0);
+    else if constexpr (P == 2) {
+        constexpr int t[3][3] = {{0, 1, 2}, {2, 4, 5}, {5, 3, 0}};
+        return t[s][m];
+    } else {
+        constexpr int t[3][4] = {{0, 1, 2, 3}, {3, 6, 8, 9}, {9, 7, 4, 0}};
+        return t[s][m];
+    }
+}
+
+// Roe flux of Numericalflux.cpp:22-166 with the reference's eigenvalue
+// "sound speed" ci = c2^2 (Numericalflux.cpp:117).  Divisions by rho are
+// taken as one reciprocal per side and the pressure uses
+// 0.5 rho q^2 = 0.5 (m_x^2 + m_y^2) / rho; both are reassociations of the
+// reference arithmetic.  Returns 0 or the reference's error number.
+__device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR)[4], double nx, double ny,
+                                        double (&F)[4]) {
+    constexpr double gamma = 1.4;
+    constexpr double gmi = gamma - 1.0;
+    const double rL = UL[0], irL = 1.0 / rL;
+    const double uL = UL[1] * irL, vL = UL[2] * irL;
+    const double unL = uL * nx + vL * ny;
+    const double pL = gmi * (UL[3] - 0.5 * (UL[1] * UL[1] + UL[2] * UL[2]) * irL);
+    const double rR = UR[0], irR = 1.0 / rR;
+    const double uR = UR[1] * irR, vR = UR[2] * irR;
+    const double unR = uR * nx + vR * ny;
+    const double pR = gmi * (UR[3] - 0.5 * (UR[1] * UR[1] + UR[2] * UR[2]) * irR);
+    int code = 0;
+    if ((pL <= 0) || (rL <= 0)) code = 1;
+    else if ((pR <= 0) || (rR <= 0)) code = 2;
+    const double rHL = UL[3] + pL, HL = rHL * irL;
+    const double rHR = UR[3] + pR, HR = rHR * irR;
+    const double du0 = UR[0] - UL[0], du1 = UR[1] - UL[1], du2 = UR[2] - UL[2], du3 = UR[3] - UL[3];
+    const double di = sqrt(rR * irL);
+    const double d1 = 1.0 / (1.0 + di);
+    const double ui = (di * uR + uL) * d1;
+    const double vi = (di * vR + vL) * d1;
+    const double Hi = (di * HR + HL) * d1;
+    const double af = 0.5 * (ui * ui + vi * vi);
+    const double ucp = ui * nx + vi * ny;
+    const double c2 = gmi * (Hi - af);
+    if (code == 0 && c2 <= 0) code = 3;
+    const double ci = c2 * c2;
+    const double ci1 = 1.0 / ci;
+    const double eps = ci * 0.1;
+    double l0 = ucp + ci, l1 = ucp - ci, l2 = ucp;
+    if ((l0 < eps) && (l0 > -eps)) l0 = 0.5 * (eps + l0 * l0 / eps);
+    if ((l1 < eps) && (l1 > -eps)) l1 = 0.5 * (eps + l1 * l1 / eps);
+    if ((l2 < eps) && (l2 > -eps)) l2 = 0.5 * (eps + l2 * l2 / eps);
+    l0 = fabs(l0); l1 = fabs(l1); l2 = fabs(l2);
+    const double s1 = 0.5 * (l0 + l1);
+    const double s2 = 0.5 * (l0 - l1);
+    const double G1 = gmi * (af * du0 - ui * du1 - vi * du2 + du3);
+    const double G2 = -ucp * du0 + du1 * nx + du2 * ny;
+    const double C1 = G1 * (s1 - l2) * ci1 * ci1 + G2 * s2 * ci1;
+    const double C2 = G1 * s2 * ci1 + G2 * (s1 - l2);
+    F[0] = 0.5 * (rL * unL + rR * unR) - 0.5 * (l2 * du0 + C1);
+    F[1] = 0.5 * ((UL[1] * unL + pL * nx) + (UR[1] * unR + pR * nx)) - 0.5 * (l2 * du1 + C1 * ui + C2 * nx);
+    F[2] = 0.5 * ((UL[2] * unL + pL * ny) + (UR[2] * unR + pR * ny)) - 0.5 * (l2 * du2 + C1 * vi + C2 * ny);
+    F[3] = 0.5 * (rHL * unL + rHR * unR) - 0.5 * (l2 * du3 + C1 * Hi + C2 * ucp);
+    return code;
+}
+
+__device__ __forceinline__ void report(unsigned long long* err, int stage, int e, int q, int s, int code) {
+    const unsigned long long key = (static_cast<unsigned long long>(stage) << 40) |
+                                   ((((static_cast<unsigned long long>(e) * 16 + q) * 4 + s) * 4) + code);
+    atomicMin(err, key);
+}
+
+// Element index with XCD-aware block order: consecutive workgroups are
+// dispatched round-robin over the 8 XCDs, so block b is remapped to logical
+// block (b % 8) * per_xcd + b / 8.  Each XCD then sweeps one contiguous band
+// of elements, and the vertical neighbours (one element row away) of the
+// band's rows are read through that XCD's own L2.
+__device__ __forceinline__ int element_index(const StageParams& prm) {
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * prm.blocks_per_xcd + (b >> 3);
+    return lb * blockDim.x + threadIdx.x;
+}
+
+// Plane access: the plane base is wavefront-uniform (scalar registers) and
+// the element offset a 32-bit byte offset, so loads and stores take the
+// global_{load,store} saddr + voffset form instead of 64-bit per-lane addresses.
+__device__ __forceinline__ const double& plane_at(const double* base, int plane, size_t ld8, unsigned off8) {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + static_cast<size_t>(plane) * ld8 + off8);
+}
+__device__ __forceinline__ double& plane_at(double* base, int plane, size_t ld8, unsigned off8) {
+    return *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + static_cast<size_t>(plane) * ld8 + off8);
+}
+
+template <int P, int NQ>
+__global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams prm) {
+    constexpr int NP = (P + 1) * (P + 2) / 2;
+    constexpr int NE = P + 1;
+    constexpr double gmi = 1.4 - 1.0;
+    const int e = element_index(prm);
+    if (e >= prm.ne) return;
+    const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
+    const unsigned oe = static_cast<unsigned>(e) * 8u;
+    const double* __restrict__ T = prm.tab;
+    const double* __restrict__ Uin = prm.Uin;
+
+    const double x0 = plane_at(prm.vtx, 0, ld8, oe), y0 = plane_at(prm.vtx, 1, ld8, oe);
+    const double x1 = plane_at(prm.vtx, 2, ld8, oe), y1 = plane_at(prm.vtx, 3, ld8, oe);
+    const double x2 = plane_at(prm.vtx, 4, ld8, oe), y2 = plane_at(prm.vtx, 5, ld8, oe);
+    unsigned onb[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) onb[s] = static_cast<unsigned>(prm.nbr[s * prm.ld + e]) * 8u;
+
+    double r[NP][4];
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[i][k] = 0.0;
+
+    // Surface term: Roe flux at the Gauss points of each side, lifted with
+    // len_s * w_q * phi_i (Evolve.cpp:242-244) and subtracted (R = V - L).
+    // Only the P+1 edge nodes of the element and of its neighbour enter.
+    const double vx[3] = {x0, x1, x2}, vy[3] = {y0, y1, y2};
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        double ue[NE][4], un[NE][4];
+#pragma unroll
+        for (int m = 0; m < NE; ++m)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ue[m][k] = plane_at(Uin, k * NP + edge_node<P>(s, m), ld8, oe);
+                un[m][k] = plane_at(Uin, k * NP + edge_node<P>(s, m), ld8, onb[s]);
+            }
+        const double dx = vx[(s + 1) % 3] - vx[s], dy = vy[(s + 1) % 3] - vy[s];
+        const double len = sqrt(dx * dx + dy * dy);
+        const double nxv = dy / len, nyv = -dx / len;
+        const double* __restrict__ tp = T + tab_tp(P) + s * NQ * NE;
+        const double* __restrict__ tm = T + tab_tm(P, NQ) + s * NQ * NE;
+        const double* __restrict__ tl = T + tab_lift(P, NQ) + s * NQ * NE;
+#pragma unroll VT_Q_UNROLL
+        for (int q = 0; q < NQ; ++q) {
+            double UL[4], UR[4], F[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                double a = 0.0, b = 0.0;
+#pragma unroll
+                for (int m = 0; m < NE; ++m) {
+                    a += tp[q * NE + m] * ue[m][k];
+                    b += tm[q * NE + m] * un[m][k];
+                }
+                UL[k] = a;
+                UR[k] = b;
+            }
+            const int code = roe_flux(UL, UR, nxv, nyv, F);
+            if (code) report(prm.err, prm.stage_id, e, q, s, code);
+#pragma unroll
+            for (int m = 0; m < NE; ++m) {
+                const double w = len * tl[q * NE + m];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) r[edge_node<P>(s, m)][k] -= w * F[k];
+            }
+        }
+    }
+
+    // Volume term: r += Sxi G^xi + Seta G^eta with G = adj(J) F (detJ * invJ = adj(J)),
+    // streaming one node's state at a time.
+    const double a00 = y2 - y0, a01 = x0 - x2, a10 = y0 - y1, a11 = x1 - x0;
+#pragma unroll VT_J_UNROLL
+    for (int j = 0; j < NP; ++j) {
+        const double rho = plane_at(Uin, 0 * NP + j, ld8, oe);
+        const double m1 = plane_at(Uin, 1 * NP + j, ld8, oe);
+        const double m2 = plane_at(Uin, 2 * NP + j, ld8, oe);
+        const double m3 = plane_at(Uin, 3 * NP + j, ld8, oe);
+        const double ir = 1.0 / rho;
+        const double uu = m1 * ir, vv = m2 * ir, E = m3 * ir;
+        const double p = rho * gmi * (E - 0.5 * (uu * uu + vv * vv));
+        const double H = E + p * ir;
+        const double fx[4] = {rho * uu, rho * uu * uu + p, rho * uu * vv, rho * uu * H};
+        const double fy[4] = {rho * vv, rho * uu * vv, rho * vv * vv + p, rho * vv * H};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double gx = a00 * fx[k] + a01 * fy[k];
+            const double gy = a10 * fx[k] + a11 * fy[k];
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                r[i][k] += T[tab_sxi(P) + i * NP + j] * gx + T[tab_seta(P) + i * NP + j] * gy;
+        }
+    }
+
+    // dU/dt = Minv_ref R / detJ, then the stage update.
+    const double detJ = (x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0);
+    const double idet = 1.0 / detJ;
+    const double dt = prm.dt;
+    constexpr double c6 = 1.0 / 6.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            double d = 0.0;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) d += T[tab_minv(P) + i * NP + j] * r[j][k];
+            d *= idet;
+            const int pl = k * NP + i;
+            switch (prm.mode) {
+            case kModeRhs: plane_at(prm.Uout, pl, ld8, oe) = d; break;
+            case kModeEuler: plane_at(prm.Uout, pl, ld8, oe) = plane_at(Uin, pl, ld8, oe) + dt * d; break;
+            case kModeRk1: {  // k1 = dt f(Un); acc = k1; U2 = Un + k1/2
+                const double kk = dt * d;
+                plane_at(prm.acc, pl, ld8, oe) = kk;
+                plane_at(prm.Uout, pl, ld8, oe) = plane_at(Uin, pl, ld8, oe) + kk * 0.5;
+            } break;
+            case kModeRk2: {  // acc = k1 + 2 k2; U3 = Un + k2/2
+                const double kk = dt * d;
+                plane_at(prm.acc, pl, ld8, oe) = plane_at(prm.acc, pl, ld8, oe) + 2.0 * kk;
+                plane_at(prm.Uout, pl, ld8, oe) = plane_at(prm.Un, pl, ld8, oe) + kk * 0.5;
+            } break;
+            case kModeRk3: {  // acc = Un + (k1 + 2 k2 + 2 k3)/6; U4 = Un + k3
+                const double kk = dt * d;
+                const double un0 = plane_at(prm.Un, pl, ld8, oe);
+                plane_at(prm.acc, pl, ld8, oe) = un0 + c6 * (plane_at(prm.acc, pl, ld8, oe) + 2.0 * kk);
+                plane_at(prm.Uout, pl, ld8, oe) = un0 + kk;
+            } break;
+            default: {  // kModeRk4: U_{n+1} = acc + k4/6
+                plane_at(prm.Uout, pl, ld8, oe) = plane_at(prm.acc, pl, ld8, oe) + c6 * (dt * d);
+            } break;
+            }
+        }
+}
+
+// Batch Roe flux, used to check the device flux function on its own.
+__global__ void flux_kernel(const double* __restrict__ in, double* __restrict__ out, int* __restrict__ codes, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* r = in + 10 * static_cast<size_t>(i);
+    double UL[4] = {r[0], r[1], r[2], r[3]}, UR[4] = {r[4], r[5], r[6], r[7]}, F[4];
+    codes[i] = roe_flux(UL, UR, r[8], r[9], F);
+    for (int k = 0; k < 4; ++k) out[4 * static_cast<size_t>(i) + k] = F[k];
+}
+
+template <int P, int NQ>
+static hipError_t launch_pq(const StageParams& prm, int grid, hipStream_t st) {
+    hipLaunchKernelGGL((stage_kernel<P, NQ>), dim3(grid), dim3(kBlock), 0, st, prm);
+    return hipGetLastError();
+}
+
+template <int P>
+static hipError_t launch_p(const StageParams& prm, int nq, int grid, hipStream_t st) {
+    switch (nq) {
+    case 1: return launch_pq<P, 1>(prm, grid, st);
+    case 2: return launch_pq<P, 2>(prm, grid, st);
+    case 3: return launch_pq<P, 3>(prm, grid, st);
+    case 4: return launch_pq<P, 4>(prm, grid, st);
+    case 5: return launch_pq<P, 5>(prm, grid, st);
+    case 6: return launch_pq<P, 6>(prm, grid, st);
+    case 7: return launch_pq<P, 7>(prm, grid, st);
+    case 8: return launch_pq<P, 8>(prm, grid, st);
+    case 9: return launch_pq<P, 9>(prm, grid, st);
+    case 10: return launch_pq<P, 10>(prm, grid, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_stage(const StageParams& prm, int p, int nq, hipStream_t st) {
+    const int grid = 8 * prm.blocks_per_xcd;
+    switch (p) {
+    case 0: return launch_p<0>(prm, nq, grid, st);
+    case 1: return launch_p<1>(prm, nq, grid, st);
+    case 2: return launch_p<2>(prm, nq, grid, st);
+    case 3: return launch_p<3>(prm, nq, grid, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(flux_kernel, dim3((n + 255) / 256), dim3(256), 0, st, in, out, codes, n);
+    return hipGetLastError();
+}
+
+}  // namespace vt
