@@ -1,0 +1,161 @@
+"""Benchmark: RK4 DOF-stage updates/s of the HIP path at p = 3, FP64.
+
+Workload (one GPU): the isentropic vortex on 512 x 512 cells (524,288
+triangles) of a 20 x 20 periodic domain, interior vertices moved by
+0.02 h U(-1,1) (seed 2), p = 3, integration_order = 8, classic RK4.  A step
+is one RK4 step over the whole mesh (four fused stage kernels); the state is
+resident in HBM before timing starts.
+
+dt = 1e-4: at h = 20/512 the configured dt = 1e-3 makes the reference
+numerics leave the physical range in step 4 (its Roe eigenvalues use
+ci = c2^2 ~ 16), after which the reference exits.  The work per step does not
+depend on dt.
+
+Prints one JSON line (rank 0).  --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RK4 DOF-stage updates/s at p=3 FP64; batched & adjoint vs plain at 1e-12"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per element per RK4 step in the 3-array form
+# (U_n, stage state, accumulator): 960 + 1600 + 1600 + 960 = 5120 B at p = 3,
+# i.e. 1280 B per element per stage launch on average.
+BYTES_PER_ELEM_STEP = 5120.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_stage_kernel_p3_512x512.txt")
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cells", type=int, default=512)
+    ap.add_argument("--dt", type=float, default=1e-4)
+    ap.add_argument("--kernel-steps", type=int, default=10, help="steps timed per stage launch for the roofline")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def traffic_from_profile(n_elem):
+    """HBM bytes per stage launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), scaled by the copy-kernel calibration, read from the
+    committed summary; None when it is absent or for another mesh size."""
+    try:
+        txt = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^bytes_per_launch_avg:\s*([0-9.eE+]+)", txt, re.M)
+    e = re.search(r"^elements:\s*(\d+)", txt, re.M)
+    if not m or not e or int(e.group(1)) != n_elem:
+        return None
+    return float(m.group(1))
+
+
+def cpu_baseline(mesh, p, order, steps):
+    """The reference's own translation units (oracle/_ref/vt_ref_run, OpenMP)
+    timed on this host over `steps` RK4 steps of the same mesh; falls back to
+    the oracle port when the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    ne = mesh.num_elements
+    npn = (p + 1) * (p + 2) // 2
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sample = "%d RK4 steps of the same %d-element mesh" % (steps, ne)
+    if os.path.exists(O.REF_RUN):
+        with tempfile.TemporaryDirectory() as d:
+            r = O.run_reference(d, dict(p=p, nx=1, ny=1, integration_order=order, simulation_time="0.2",
+                                        number_time_steps=2000), steps,
+                                mesh=(mesh.element_coordinates, mesh.elements_at_boundary, mesh.element_type),
+                                threads=threads, no_state=True)
+        sec = float(re.search(r"stepping_sec=([0-9.]+)", r["stdout"]).group(1))
+        kind = "reference"
+    else:
+        s = O.Solver(mesh.element_coordinates, mesh.elements_at_boundary, mesh.element_type, p, order)
+        s.init_vortex(0)
+        t0 = time.perf_counter()
+        s.run(1, 1e-4, steps)
+        sec = time.perf_counter() - t0
+        kind = "port"
+    return {"value": ne * npn * 4 * 4 * steps / sec, "unit": "DOF-stage updates/s", "cores": threads,
+            "kind": kind, "sample": sample, "seconds": round(sec, 3)}
+
+
+def main():
+    a = parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import __graft_entry__ as ge
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    vt = ge.load_package()
+    p, order, n = 3, 8, a.cells
+    mesh = vt.generate_mesh_seeded(n, n, 20.0, 20.0, 0.02, 2)
+    s = vt.Solver(mesh, p, order)
+    s.initialize_hydrodinamics(0)
+    ne, npn = s.ne, s.np
+
+    s.rk4(a.dt, a.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.rk4(a.dt, a.steps)          # graph replays on the library stream; returns after it drains
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    U = s.state()
+    assert np.isfinite(U).all()
+
+    # Per-launch duration of the stage kernel, HIP events on the library's stream.
+    stage_ms = s.stage_kernel_times(a.dt, a.kernel_steps)
+    avg_launch_s = float(np.mean(stage_ms)) * 1e-3
+    bytes_per_launch = ne * BYTES_PER_ELEM_STEP / 4
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = traffic_from_profile(ne)
+
+    dof_stage = ne * npn * 4 * 4
+    line = {
+        "metric": METRIC,
+        "value": dof_stage * a.steps / wall,
+        "unit": "DOF-stage updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "isentropic vortex, %dx%d cells (%d triangles), 20x20 periodic domain, seeded perturbed "
+                               "mesh (0.02 h, seed 2), p=3, integration_order=8, RK4" % (n, n, ne),
+                   "elements": ne, "p": p, "integration_order": order, "dt": a.dt,
+                   "dof_stage_per_step": dof_stage},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "vt::stage_kernel<3,5> (4 launches per step)",
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": avg_launch_s * 1e3, "stage_ms": [float(x) for x in stage_ms]},
+    }
+    s.close()
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(mesh, p, order, a.cpu_steps)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -231,7 +231,7 @@ class Solver:
         return lib().oro_run(self.h, method, dt, nsteps)
 
     def last_error(self):
-        info = np.zeros(4, dtype=np.int32)
+        info = np.zeros(6, dtype=np.int32)
         lib().oro_last_error(self.h, _i(info))
         return tuple(int(x) for x in info)
 
@@ -250,7 +250,7 @@ def write_input(path, p=3, nx=8, ny=8, domain_x="10.0", domain_y="10.0", perturb
                 (write_every_steps, U_initialization_type, stepping_method))
 
 
-def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, threads=None):
+def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, threads=None, no_state=False):
     """Runs oracle/_ref/vt_ref_run (the reference's own TUs) in workdir; returns a dict of arrays."""
     if not os.path.exists(REF_RUN):
         raise FileNotFoundError(REF_RUN)
@@ -268,6 +268,8 @@ def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, thr
     cmd = [REF_RUN, "run", inp, out, "--minv", minv, "--stiff", stiff, "--steps", str(steps)]
     if rhs_at_end:
         cmd.append("--rhs-at-end")
+    if no_state:
+        cmd.append("--no-state")
     if mesh is not None:
         v, b, t = mesh
         ne = v.shape[0]
@@ -284,6 +286,8 @@ def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, thr
     r = subprocess.run(cmd, cwd=workdir, capture_output=True, text=True, env=env)
     if r.returncode != 0:
         raise RuntimeError("reference run failed: %s %s" % (r.stdout[-2000:], r.stderr[-2000:]))
+    if no_state:
+        return {"stdout": r.stdout}
     d = np.fromfile(out)
     nx, ny = input_kwargs.get("nx", 8), input_kwargs.get("ny", 8)
     ne = mesh[0].shape[0] if mesh is not None else 2 * nx * ny

@@ -315,7 +315,8 @@ struct oro_solver {
     double plus[3][MAXNP][MAXNQ], minus[3][MAXNP][MAXNQ];
     double *U, *F, *time, *dU;
     int order;
-    int err[4];
+    int err[6];        /* elem, q, side, code, step, stage of the first failure */
+    int cur_step, cur_stage;
 };
 
 #define GEO_JAC 0
@@ -685,7 +686,9 @@ static uint64_t rhs_element(const oro_solver *s, int e, double *dU) {
 }
 
 static void set_err(oro_solver *s, uint64_t key) {
-    if (key == UINT64_MAX) return;
+    if (key == UINT64_MAX || s->err[3] != 0) return; /* keep the first failure */
+    s->err[4] = s->cur_step;
+    s->err[5] = s->cur_stage;
     uint64_t t = key;
     s->err[3] = (int)(t % 4); t /= 4;
     s->err[2] = (int)(t % 4); t /= 4;
@@ -696,6 +699,7 @@ static void set_err(oro_solver *s, uint64_t key) {
 /* Timestepping.cpp:18-36 */
 static int evolve(oro_solver *s) {
     uint64_t best = UINT64_MAX;
+    s->cur_stage++;
     const int ne = s->ne;
 #ifdef _OPENMP
     int nt = g_threads > 0 ? g_threads : omp_get_max_threads();
@@ -710,6 +714,7 @@ static int evolve(oro_solver *s) {
 }
 
 int oro_evolve_elem(oro_solver *s, double *dUdt) {
+    s->cur_stage = 0;
     int rc = evolve(s);
     if (dUdt) memcpy(dUdt, s->dU, sizeof(double) * s->ne * s->np * 4);
     return rc;
@@ -717,6 +722,7 @@ int oro_evolve_elem(oro_solver *s, double *dUdt) {
 
 /* Timestepping.cpp:39-82 */
 int oro_forward_euler(oro_solver *s, double t_step) {
+    s->cur_stage = 0;
     int rc = evolve(s);
     size_t n = (size_t)s->ne * s->np;
     for (size_t i = 0; i < n; ++i) {
@@ -736,6 +742,7 @@ int oro_rk4(oro_solver *s, double t_step) {
     memcpy(U, s->U, sizeof(double) * n);
     int rc = 0, r;
     size_t nn = n / 4;
+    s->cur_stage = 0;
     r = evolve(s); if (!rc) rc = r;
     for (size_t i = 0; i < nn; ++i) {
         for (int k = 0; k < 4; ++k) {
@@ -774,16 +781,19 @@ int oro_rk4(oro_solver *s, double t_step) {
     return rc;
 }
 
+/* Stops after the step in which the first non-physical state occurs, where
+ * the reference exits (Numericalflux.cpp:62,87,123). */
 int oro_run(oro_solver *s, int method, double dt, int nsteps) {
-    int rc = 0;
+    memset(s->err, 0, sizeof(s->err));
     for (int a = 0; a < nsteps; ++a) {
         int r;
+        s->cur_step = a + 1;
         if (method == 0) r = oro_forward_euler(s, dt);
         else if (method == 1) r = oro_rk4(s, dt);
         else return ORO_ERR_ARG;
-        if (r && !rc) rc = r;
+        if (r) return r;
     }
-    return rc;
+    return 0;
 }
 
 /* plottools/l2stateerror.py:31-99 with the Dunavant order-10 table of

@@ -87,10 +87,12 @@ int oro_evolve_elem(oro_solver *s, double *dUdt);
 /* forward_euler (Timestepping.cpp:39-82) and rk4 (Timestepping.cpp:85-305). */
 int oro_forward_euler(oro_solver *s, double dt);
 int oro_rk4(oro_solver *s, double dt);
-/* method 0 forward Euler, 1 RK4 (main.cpp:154-169). Returns first error code. */
+/* method 0 forward Euler, 1 RK4 (main.cpp:154-169).  Returns the first error
+ * code and stops after that step. */
 int oro_run(oro_solver *s, int method, double dt, int nsteps);
-/* Last error location: elem, quadrature point, side (0..2), code. */
-void oro_last_error(const oro_solver *s, int *info /*[4]*/);
+/* First error location: elem, quadrature point, side (0..2), code, step
+ * (1-based within the last oro_run), stage (1..4). */
+void oro_last_error(const oro_solver *s, int *info /*[6]*/);
 
 /* Roe flux (Numericalflux.cpp:13-156). Returns 0 or the error code. */
 int oro_numerical_flux(const double *UL, const double *UR, const double *n, double *flux);
