@@ -75,6 +75,15 @@ int vt_quadrature(int integration_order, int *nq_line, double *line_xw, int *n_a
  * constructor with evaluate_basis_in_quadrature_poits (Evolve.cpp:24-132). */
 int vt_create(int n_elem, const double *vertices, const int *neighbors, const int *type, int p,
               int integration_order, vt_ctx **out);
+/* Same, with the operators the reference passes to each element instead of an
+ * integration order: the line rule line_xw [nq][2] ({sigma, weight}, the
+ * Evolve_element constructor argument, Evolve.H:15), the reference inverse mass
+ * matrix Minv_ref [Np][Np] and stiffness matrices S_ref [2][Np][Np] (the
+ * arguments of Element::build_mass_matrix_inverse / build_stiffness_matrix,
+ * Element.H:49-50).  Used by the C++ drop-in classes. */
+int vt_create_with_operators(int n_elem, const double *vertices, const int *neighbors, const int *type, int p,
+                             int nq, const double *line_xw, const double *Minv_ref, const double *S_ref,
+                             vt_ctx **out);
 void vt_destroy(vt_ctx *ctx);
 int vt_num_elements(const vt_ctx *ctx);
 int vt_num_nodes(const vt_ctx *ctx);

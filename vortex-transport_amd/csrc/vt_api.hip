@@ -264,13 +264,10 @@ int vt_quadrature(int order, int* nq_line, double* line_xw, int* n_area, double*
     return VT_OK;
 }
 
-int vt_create(int n, const double* vertices, const int* neighbors, const int* type, int p, int order, vt_ctx** out) {
-    if (!out) return VT_ERR_ARG;
-    *out = nullptr;
-    if (n < 1 || !vertices || !neighbors || !type) return VT_ERR_ARG;
-    if (p < 0 || p > vt::kMaxP) return VT_ERR_UNSUPPORTED_P;
-    vt::OperatorTable tab;
-    if (vt::line_rule(order, nullptr) < 0 || !vt::build_operator_table(p, order, &tab)) return VT_ERR_QUAD;
+namespace {
+
+int create_ctx(int n, const double* vertices, const int* neighbors, const int* type, int p, const vt::OperatorTable& tab,
+               vt_ctx** out) {
     for (int e = 0; e < n; ++e) {
         if (type[e] != 0 && type[e] != 1) return VT_ERR_ARG;
         for (int s = 0; s < 3; ++s)
@@ -280,7 +277,7 @@ int vt_create(int n, const double* vertices, const int* neighbors, const int* ty
         static_cast<long long>(n) * 256 >= (1ll << 40))
         return VT_ERR_ARG;  // 32-bit byte offsets and the 40-bit error key
     vt_ctx* c = new vt_ctx();
-    c->ne = n; c->ld = plane_stride(n); c->p = p; c->np = vt::num_nodes(p); c->nq = tab.nq; c->order = order;
+    c->ne = n; c->ld = plane_stride(n); c->p = p; c->np = vt::num_nodes(p); c->nq = tab.nq; c->order = tab.order;
     c->vertices.assign(vertices, vertices + 6 * static_cast<size_t>(n));
     auto fail = [&](int rc) { vt_destroy(c); return rc; };
     if (!hip_ok(hipGetDevice(&c->device), "hipGetDevice")) return fail(VT_ERR_HIP);
@@ -312,6 +309,31 @@ int vt_create(int n, const double* vertices, const int* neighbors, const int* ty
     return VT_OK;
 }
 
+}  // namespace
+
+int vt_create(int n, const double* vertices, const int* neighbors, const int* type, int p, int order, vt_ctx** out) {
+    if (!out) return VT_ERR_ARG;
+    *out = nullptr;
+    if (n < 1 || !vertices || !neighbors || !type) return VT_ERR_ARG;
+    if (p < 0 || p > vt::kMaxP) return VT_ERR_UNSUPPORTED_P;
+    vt::OperatorTable tab;
+    if (vt::line_rule(order, nullptr) < 0 || !vt::build_operator_table(p, order, &tab)) return VT_ERR_QUAD;
+    return create_ctx(n, vertices, neighbors, type, p, tab, out);
+}
+
+int vt_create_with_operators(int n, const double* vertices, const int* neighbors, const int* type, int p, int nq,
+                             const double* line_xw, const double* Minv_ref, const double* S_ref, vt_ctx** out) {
+    if (!out) return VT_ERR_ARG;
+    *out = nullptr;
+    if (n < 1 || !vertices || !neighbors || !type || !line_xw || !Minv_ref || !S_ref) return VT_ERR_ARG;
+    if (p < 0 || p > vt::kMaxP) return VT_ERR_UNSUPPORTED_P;
+    if (nq < 1 || nq > vt::kMaxNq) return VT_ERR_QUAD;
+    vt::OperatorTable tab;
+    if (!vt::build_operator_table(p, nq, line_xw, Minv_ref, S_ref, &tab)) return VT_ERR_ARG;
+    tab.order = -1;
+    return create_ctx(n, vertices, neighbors, type, p, tab, out);
+}
+
 void vt_destroy(vt_ctx* c) {
     if (!c) return;
     for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
@@ -330,6 +352,7 @@ int vt_num_nodes(const vt_ctx* c) { return c ? c->np : 0; }
 
 int vt_init_vortex(vt_ctx* c, int init_type, double epsilon, double x0, double y0) {
     if (!c) return VT_ERR_ARG;
+    if (init_type == 1 && c->order < 0) return VT_ERR_QUAD;  // no area rule known
     std::vector<double> U(static_cast<size_t>(c->ne) * c->np * 4);
     if (!vt::initial_state(c->ne, c->vertices.data(), c->p, c->order, init_type, epsilon, x0, y0, c->time, U.data()))
         return VT_ERR_INIT_TYPE;

@@ -31,6 +31,9 @@
 #ifndef VT_J_UNROLL
 #define VT_J_UNROLL 1
 #endif
+#ifndef VT_PREFETCH_NEXT
+#define VT_PREFETCH_NEXT 0
+#endif
 #ifndef VT_MIN_WAVES
 #define VT_MIN_WAVES 2
 #endif
@@ -134,13 +137,24 @@ template <int P, int NQ>
 __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams prm) {
     constexpr int NP = (P + 1) * (P + 2) / 2;
     constexpr int NE = P + 1;
+    constexpr int NV = 4 * NP;
     constexpr double gmi = 1.4 - 1.0;
+    // The element's own stage state, one LDS column per thread (value v of
+    // thread t at su[v * kBlock + t]): read once from HBM with all loads in
+    // flight together, then re-read by the volume term, the three traces and
+    // the stage update without further global traffic.  Columns are private,
+    // so no barrier is needed.
+    __shared__ double su[NV * kBlock];
     const int e = element_index(prm);
     if (e >= prm.ne) return;
     const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
     const unsigned oe = static_cast<unsigned>(e) * 8u;
     const double* __restrict__ T = prm.tab;
     const double* __restrict__ Uin = prm.Uin;
+    double* my = su + threadIdx.x;
+
+#pragma unroll
+    for (int v = 0; v < NV; ++v) my[v * kBlock] = plane_at(Uin, v, ld8, oe);
 
     const double x0 = plane_at(prm.vtx, 0, ld8, oe), y0 = plane_at(prm.vtx, 1, ld8, oe);
     const double x1 = plane_at(prm.vtx, 2, ld8, oe), y1 = plane_at(prm.vtx, 3, ld8, oe);
@@ -149,66 +163,26 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
 #pragma unroll
     for (int s = 0; s < 3; ++s) onb[s] = static_cast<unsigned>(prm.nbr[s * prm.ld + e]) * 8u;
 
+    // Edge-node values of the neighbour across side 1, in flight during the volume term.
+    double un[NE][4];
+#pragma unroll
+    for (int m = 0; m < NE; ++m)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) un[m][k] = plane_at(Uin, k * NP + edge_node<P>(0, m), ld8, onb[0]);
+
+    // Volume term: r = Sxi G^xi + Seta G^eta with G = adj(J) F (detJ * invJ = adj(J)).
+    const double a00 = y2 - y0, a01 = x0 - x2, a10 = y0 - y1, a11 = x1 - x0;
     double r[NP][4];
 #pragma unroll
     for (int i = 0; i < NP; ++i)
 #pragma unroll
         for (int k = 0; k < 4; ++k) r[i][k] = 0.0;
-
-    // Surface term: Roe flux at the Gauss points of each side, lifted with
-    // len_s * w_q * phi_i (Evolve.cpp:242-244) and subtracted (R = V - L).
-    // Only the P+1 edge nodes of the element and of its neighbour enter.
-    const double vx[3] = {x0, x1, x2}, vy[3] = {y0, y1, y2};
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-        double ue[NE][4], un[NE][4];
-#pragma unroll
-        for (int m = 0; m < NE; ++m)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                ue[m][k] = plane_at(Uin, k * NP + edge_node<P>(s, m), ld8, oe);
-                un[m][k] = plane_at(Uin, k * NP + edge_node<P>(s, m), ld8, onb[s]);
-            }
-        const double dx = vx[(s + 1) % 3] - vx[s], dy = vy[(s + 1) % 3] - vy[s];
-        const double len = sqrt(dx * dx + dy * dy);
-        const double nxv = dy / len, nyv = -dx / len;
-        const double* __restrict__ tp = T + tab_tp(P) + s * NQ * NE;
-        const double* __restrict__ tm = T + tab_tm(P, NQ) + s * NQ * NE;
-        const double* __restrict__ tl = T + tab_lift(P, NQ) + s * NQ * NE;
-#pragma unroll VT_Q_UNROLL
-        for (int q = 0; q < NQ; ++q) {
-            double UL[4], UR[4], F[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                double a = 0.0, b = 0.0;
-#pragma unroll
-                for (int m = 0; m < NE; ++m) {
-                    a += tp[q * NE + m] * ue[m][k];
-                    b += tm[q * NE + m] * un[m][k];
-                }
-                UL[k] = a;
-                UR[k] = b;
-            }
-            const int code = roe_flux(UL, UR, nxv, nyv, F);
-            if (code) report(prm.err, prm.stage_id, e, q, s, code);
-#pragma unroll
-            for (int m = 0; m < NE; ++m) {
-                const double w = len * tl[q * NE + m];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) r[edge_node<P>(s, m)][k] -= w * F[k];
-            }
-        }
-    }
-
-    // Volume term: r += Sxi G^xi + Seta G^eta with G = adj(J) F (detJ * invJ = adj(J)),
-    // streaming one node's state at a time.
-    const double a00 = y2 - y0, a01 = x0 - x2, a10 = y0 - y1, a11 = x1 - x0;
 #pragma unroll VT_J_UNROLL
     for (int j = 0; j < NP; ++j) {
-        const double rho = plane_at(Uin, 0 * NP + j, ld8, oe);
-        const double m1 = plane_at(Uin, 1 * NP + j, ld8, oe);
-        const double m2 = plane_at(Uin, 2 * NP + j, ld8, oe);
-        const double m3 = plane_at(Uin, 3 * NP + j, ld8, oe);
+        const double rho = my[(0 * NP + j) * kBlock];
+        const double m1 = my[(1 * NP + j) * kBlock];
+        const double m2 = my[(2 * NP + j) * kBlock];
+        const double m3 = my[(3 * NP + j) * kBlock];
         const double ir = 1.0 / rho;
         const double uu = m1 * ir, vv = m2 * ir, E = m3 * ir;
         const double p = rho * gmi * (E - 0.5 * (uu * uu + vv * vv));
@@ -225,44 +199,134 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
         }
     }
 
-    // dU/dt = Minv_ref R / detJ, then the stage update.
+    // Surface term: Roe flux at the Gauss points of each side, lifted with
+    // len_s * w_q * phi_i (Evolve.cpp:242-244) and subtracted (R = V - L).
+    // Only the P+1 edge nodes of the element and of its neighbour enter; the
+    // next side's neighbour values are requested before this side is computed.
+    const double vx[3] = {x0, x1, x2}, vy[3] = {y0, y1, y2};
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        double nxt[NE][4];
+        if (VT_PREFETCH_NEXT && s < 2) {
+#pragma unroll
+            for (int m = 0; m < NE; ++m)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) nxt[m][k] = plane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
+        }
+        const double dx = vx[(s + 1) % 3] - vx[s], dy = vy[(s + 1) % 3] - vy[s];
+        const double len = sqrt(dx * dx + dy * dy);
+        const double nxv = dy / len, nyv = -dx / len;
+        const double* __restrict__ tp = T + tab_tp(P) + s * NQ * NE;
+        const double* __restrict__ tm = T + tab_tm(P, NQ) + s * NQ * NE;
+        const double* __restrict__ tl = T + tab_lift(P, NQ) + s * NQ * NE;
+#pragma unroll VT_Q_UNROLL
+        for (int q = 0; q < NQ; ++q) {
+            double UL[4], UR[4], F[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                double a = 0.0, b = 0.0;
+#pragma unroll
+                for (int m = 0; m < NE; ++m) {
+                    a += tp[q * NE + m] * my[(k * NP + edge_node<P>(s, m)) * kBlock];
+                    b += tm[q * NE + m] * un[m][k];
+                }
+                UL[k] = a;
+                UR[k] = b;
+            }
+            const int code = roe_flux(UL, UR, nxv, nyv, F);
+            if (code) report(prm.err, prm.stage_id, e, q, s, code);
+#pragma unroll
+            for (int m = 0; m < NE; ++m) {
+                const double w = len * tl[q * NE + m];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) r[edge_node<P>(s, m)][k] -= w * F[k];
+            }
+        }
+        if (s < 2) {
+#pragma unroll
+            for (int m = 0; m < NE; ++m)
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    un[m][k] = VT_PREFETCH_NEXT ? nxt[m][k] : plane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
+        }
+    }
+
+    // dU/dt = Minv_ref R / detJ, then the stage update.  One uniform branch
+    // per mode, so that the mode's loads of acc / Un are all issued before the
+    // Minv products that hide their latency.
     const double detJ = (x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0);
     const double idet = 1.0 / detJ;
     const double dt = prm.dt;
     constexpr double c6 = 1.0 / 6.0;
+    auto dudt = [&](int i, int k) {
+        double d = 0.0;
 #pragma unroll
-    for (int i = 0; i < NP; ++i)
+        for (int j = 0; j < NP; ++j) d += T[tab_minv(P) + i * NP + j] * r[j][k];
+        return d * idet;
+    };
+    const int mode = prm.mode;
+    if (mode == kModeRk2 || mode == kModeRk3) {
+        // Two halves of the variables: the second half's loads are issued once
+        // the first half's registers are free.
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            double d = 0.0;
+        for (int h = 0; h < 2; ++h) {
+            double ac[2 * NP], u0[2 * NP];
 #pragma unroll
-            for (int j = 0; j < NP; ++j) d += T[tab_minv(P) + i * NP + j] * r[j][k];
-            d *= idet;
-            const int pl = k * NP + i;
-            switch (prm.mode) {
-            case kModeRhs: plane_at(prm.Uout, pl, ld8, oe) = d; break;
-            case kModeEuler: plane_at(prm.Uout, pl, ld8, oe) = plane_at(Uin, pl, ld8, oe) + dt * d; break;
-            case kModeRk1: {  // k1 = dt f(Un); acc = k1; U2 = Un + k1/2
-                const double kk = dt * d;
-                plane_at(prm.acc, pl, ld8, oe) = kk;
-                plane_at(prm.Uout, pl, ld8, oe) = plane_at(Uin, pl, ld8, oe) + kk * 0.5;
-            } break;
-            case kModeRk2: {  // acc = k1 + 2 k2; U3 = Un + k2/2
-                const double kk = dt * d;
-                plane_at(prm.acc, pl, ld8, oe) = plane_at(prm.acc, pl, ld8, oe) + 2.0 * kk;
-                plane_at(prm.Uout, pl, ld8, oe) = plane_at(prm.Un, pl, ld8, oe) + kk * 0.5;
-            } break;
-            case kModeRk3: {  // acc = Un + (k1 + 2 k2 + 2 k3)/6; U4 = Un + k3
-                const double kk = dt * d;
-                const double un0 = plane_at(prm.Un, pl, ld8, oe);
-                plane_at(prm.acc, pl, ld8, oe) = un0 + c6 * (plane_at(prm.acc, pl, ld8, oe) + 2.0 * kk);
-                plane_at(prm.Uout, pl, ld8, oe) = un0 + kk;
-            } break;
-            default: {  // kModeRk4: U_{n+1} = acc + k4/6
-                plane_at(prm.Uout, pl, ld8, oe) = plane_at(prm.acc, pl, ld8, oe) + c6 * (dt * d);
-            } break;
+            for (int t = 0; t < 2 * NP; ++t) {
+                ac[t] = plane_at(prm.acc, 2 * h * NP + t, ld8, oe);
+                u0[t] = plane_at(prm.Un, 2 * h * NP + t, ld8, oe);
             }
+#pragma unroll
+            for (int kk2 = 0; kk2 < 2; ++kk2)
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    const int k = 2 * h + kk2, t = kk2 * NP + i, v = k * NP + i;
+                    const double kk = dt * dudt(i, k);
+                    if (mode == kModeRk2) {  // acc = k1 + 2 k2; U3 = Un + k2/2
+                        plane_at(prm.acc, v, ld8, oe) = ac[t] + 2.0 * kk;
+                        plane_at(prm.Uout, v, ld8, oe) = u0[t] + kk * 0.5;
+                    } else {  // acc = Un + (k1 + 2 k2 + 2 k3)/6; U4 = Un + k3
+                        plane_at(prm.acc, v, ld8, oe) = u0[t] + c6 * (ac[t] + 2.0 * kk);
+                        plane_at(prm.Uout, v, ld8, oe) = u0[t] + kk;
+                    }
+                }
+            __builtin_amdgcn_sched_barrier(0);
         }
+    } else if (mode == kModeRk4) {  // U_{n+1} = acc + k4/6
+        double ac[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ac[v] = plane_at(prm.acc, v, ld8, oe);
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int v = k * NP + i;
+                plane_at(prm.Uout, v, ld8, oe) = ac[v] + c6 * (dt * dudt(i, k));
+            }
+    } else if (mode == kModeRk1) {  // k1 = dt f(Un); acc = k1; U2 = Un + k1/2
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int v = k * NP + i;
+                const double kk = dt * dudt(i, k);
+                plane_at(prm.acc, v, ld8, oe) = kk;
+                plane_at(prm.Uout, v, ld8, oe) = my[v * kBlock] + kk * 0.5;
+            }
+    } else if (mode == kModeEuler) {  // U + dt dU/dt
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int v = k * NP + i;
+                plane_at(prm.Uout, v, ld8, oe) = my[v * kBlock] + dt * dudt(i, k);
+            }
+    } else {  // kModeRhs
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) plane_at(prm.Uout, k * NP + i, ld8, oe) = dudt(i, k);
+    }
 }
 
 // Batch Roe flux, used to check the device flux function on its own.

@@ -362,18 +362,15 @@ const int* edge_nodes(int p, int side) {
 
 // Edge tables (Evolve.cpp:93-131): plus points (sigma,0), (1-sigma,sigma),
 // (0,1-sigma); minus points (1-sigma,0), (sigma,1-sigma), (0,sigma).
-bool build_operator_table(int p, int order, OperatorTable* out) {
-    std::vector<double> xw;
-    const int nq = line_rule(order, &xw);
-    if (p < 0 || p > kMaxP || nq < 0 || area_rule(order, nullptr) < 0) return false;
+bool build_operator_table(int p, int nq, const double* xw, const double* Minv, const double* S, OperatorTable* out) {
+    if (p < 0 || p > kMaxP || nq < 1 || nq > kMaxNq) return false;
     const int np = num_nodes(p), ne1 = p + 1;
     OperatorTable& t = *out;
-    t.p = p; t.np = np; t.nq = nq; t.order = order;
+    t.p = p; t.np = np; t.nq = nq;
     t.data.assign(tab_size(p, nq), 0.0);
-    std::vector<double> S(2 * np * np);
-    if (!reference_operators(p, order, nullptr, t.data.data() + tab_minv(p), S.data())) return false;
-    std::memcpy(t.data.data() + tab_sxi(p), S.data(), sizeof(double) * np * np);
-    std::memcpy(t.data.data() + tab_seta(p), S.data() + np * np, sizeof(double) * np * np);
+    std::memcpy(t.data.data() + tab_minv(p), Minv, sizeof(double) * np * np);
+    std::memcpy(t.data.data() + tab_sxi(p), S, sizeof(double) * np * np);
+    std::memcpy(t.data.data() + tab_seta(p), S + np * np, sizeof(double) * np * np);
     t.max_offedge = 0;
     for (int q = 0; q < nq; ++q) {
         const double sg = xw[2 * q], w = xw[2 * q + 1];
@@ -397,6 +394,18 @@ bool build_operator_table(int p, int order, OperatorTable* out) {
             }
         }
     }
+    return true;
+}
+
+bool build_operator_table(int p, int order, OperatorTable* out) {
+    std::vector<double> xw;
+    const int nq = line_rule(order, &xw);
+    if (p < 0 || p > kMaxP || nq < 0 || area_rule(order, nullptr) < 0) return false;
+    const int np = num_nodes(p);
+    std::vector<double> Minv(np * np), S(2 * np * np);
+    if (!reference_operators(p, order, nullptr, Minv.data(), S.data())) return false;
+    if (!build_operator_table(p, nq, xw.data(), Minv.data(), S.data(), out)) return false;
+    out->order = order;
     return true;
 }
 

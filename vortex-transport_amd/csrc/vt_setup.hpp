@@ -72,5 +72,7 @@ struct OperatorTable {
     double max_offedge = 0;    // largest |phi| dropped by the edge-node restriction
 };
 bool build_operator_table(int p, int order, OperatorTable* out);
+// Same from explicit operators: line rule xw [nq][2], Minv [Np][Np], S [2][Np][Np].
+bool build_operator_table(int p, int nq, const double* xw, const double* Minv, const double* S, OperatorTable* out);
 
 }  // namespace vt
