@@ -46,7 +46,22 @@ def parse_args():
     ap.add_argument("--kernel-steps", type=int, default=10, help="steps timed per stage launch for the roofline")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="for rocprofv3 --pmc: 2 steps of per-launch stage kernels, then a plain device copy "
+                         "of 2^27 doubles (1 GiB read + 1 GiB written) to calibrate FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+def profile_pass(s, dt):
+    import torch
+    s.stage_kernel_times(dt, 2)
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_pass": True, "stage_launches": 8, "copy_bytes_read": a.numel() * 8,
+                      "copy_bytes_written": b.numel() * 8, "elements": s.ne}))
 
 
 def traffic_from_profile(n_elem):
@@ -110,6 +125,10 @@ def main():
     s = vt.Solver(mesh, p, order)
     s.initialize_hydrodinamics(0)
     ne, npn = s.ne, s.np
+    if a.profile_pass:
+        s.rk4(a.dt, 1)
+        profile_pass(s, a.dt)
+        return
 
     s.rk4(a.dt, a.warmup)
     torch.cuda.synchronize()

@@ -53,20 +53,32 @@ __device__ __forceinline__ constexpr int edge_node(int s, int m) {
     }
 }
 
+// Reciprocal from the hardware estimate (v_rcp_f64) refined by two Newton
+// steps (~0.5 ulp, no special-value handling: the callers' operands are
+// positive normal numbers or the state is already reported non-physical).
+__device__ __forceinline__ double fast_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // Roe flux of Numericalflux.cpp:22-166 with the reference's eigenvalue
-// "sound speed" ci = c2^2 (Numericalflux.cpp:117).  Divisions by rho are
-// taken as one reciprocal per side and the pressure uses
-// 0.5 rho q^2 = 0.5 (m_x^2 + m_y^2) / rho; both are reassociations of the
-// reference arithmetic.  Returns 0 or the reference's error number.
+// "sound speed" ci = c2^2 (Numericalflux.cpp:117).  Divisions are taken as
+// refined reciprocals (one per density, 1/(1+di), 1/ci, and 1/eps = 10/ci),
+// and the pressure uses 0.5 rho q^2 = 0.5 (m_x^2 + m_y^2) / rho; all are
+// reassociations of the reference arithmetic at the ulp level.  Returns 0 or
+// the reference's error number.
 __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR)[4], double nx, double ny,
                                         double (&F)[4]) {
     constexpr double gamma = 1.4;
     constexpr double gmi = gamma - 1.0;
-    const double rL = UL[0], irL = 1.0 / rL;
+    const double rL = UL[0], irL = fast_rcp(rL);
     const double uL = UL[1] * irL, vL = UL[2] * irL;
     const double unL = uL * nx + vL * ny;
     const double pL = gmi * (UL[3] - 0.5 * (UL[1] * UL[1] + UL[2] * UL[2]) * irL);
-    const double rR = UR[0], irR = 1.0 / rR;
+    const double rR = UR[0], irR = fast_rcp(rR);
     const double uR = UR[1] * irR, vR = UR[2] * irR;
     const double unR = uR * nx + vR * ny;
     const double pR = gmi * (UR[3] - 0.5 * (UR[1] * UR[1] + UR[2] * UR[2]) * irR);
@@ -77,7 +89,7 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
     const double rHR = UR[3] + pR, HR = rHR * irR;
     const double du0 = UR[0] - UL[0], du1 = UR[1] - UL[1], du2 = UR[2] - UL[2], du3 = UR[3] - UL[3];
     const double di = sqrt(rR * irL);
-    const double d1 = 1.0 / (1.0 + di);
+    const double d1 = fast_rcp(1.0 + di);
     const double ui = (di * uR + uL) * d1;
     const double vi = (di * vR + vL) * d1;
     const double Hi = (di * HR + HL) * d1;
@@ -86,12 +98,12 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
     const double c2 = gmi * (Hi - af);
     if (code == 0 && c2 <= 0) code = 3;
     const double ci = c2 * c2;
-    const double ci1 = 1.0 / ci;
-    const double eps = ci * 0.1;
+    const double ci1 = fast_rcp(ci);
+    const double eps = ci * 0.1, ieps = 10.0 * ci1;
     double l0 = ucp + ci, l1 = ucp - ci, l2 = ucp;
-    if ((l0 < eps) && (l0 > -eps)) l0 = 0.5 * (eps + l0 * l0 / eps);
-    if ((l1 < eps) && (l1 > -eps)) l1 = 0.5 * (eps + l1 * l1 / eps);
-    if ((l2 < eps) && (l2 > -eps)) l2 = 0.5 * (eps + l2 * l2 / eps);
+    if ((l0 < eps) && (l0 > -eps)) l0 = 0.5 * (eps + l0 * l0 * ieps);
+    if ((l1 < eps) && (l1 > -eps)) l1 = 0.5 * (eps + l1 * l1 * ieps);
+    if ((l2 < eps) && (l2 > -eps)) l2 = 0.5 * (eps + l2 * l2 * ieps);
     l0 = fabs(l0); l1 = fabs(l1); l2 = fabs(l2);
     const double s1 = 0.5 * (l0 + l1);
     const double s2 = 0.5 * (l0 - l1);
@@ -112,15 +124,14 @@ __device__ __forceinline__ void report(unsigned long long* err, int stage, int e
     atomicMin(err, key);
 }
 
-// Element index with XCD-aware block order: consecutive workgroups are
-// dispatched round-robin over the 8 XCDs, so block b is remapped to logical
-// block (b % 8) * per_xcd + b / 8.  Each XCD then sweeps one contiguous band
-// of elements, and the vertical neighbours (one element row away) of the
-// band's rows are read through that XCD's own L2.
-__device__ __forceinline__ int element_index(const StageParams& prm) {
+// XCD-aware block order: consecutive workgroups are dispatched round-robin
+// over the 8 XCDs, so block b is remapped to logical block
+// (b % 8) * per_xcd + b / 8.  Each XCD then sweeps one contiguous band of
+// elements, and the vertical neighbours (one element row away) of the band's
+// rows are read through that XCD's own L2.
+__device__ __forceinline__ int logical_block(const StageParams& prm) {
     const int b = blockIdx.x;
-    const int lb = (b & 7) * prm.blocks_per_xcd + (b >> 3);
-    return lb * blockDim.x + threadIdx.x;
+    return (b & 7) * prm.blocks_per_xcd + (b >> 3);
 }
 
 // Plane access: the plane base is wavefront-uniform (scalar registers) and
@@ -139,36 +150,56 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
     constexpr int NE = P + 1;
     constexpr int NV = 4 * NP;
     constexpr double gmi = 1.4 - 1.0;
-    // The element's own stage state, one LDS column per thread (value v of
-    // thread t at su[v * kBlock + t]): read once from HBM with all loads in
-    // flight together, then re-read by the volume term, the three traces and
-    // the stage update without further global traffic.  Columns are private,
-    // so no barrier is needed.
+    // The block's stage state, one LDS column per thread (value v of thread t
+    // at su[v * kBlock + t]): read once from HBM with all loads in flight
+    // together, then re-read by the volume term, the traces of the element and
+    // of its in-block neighbours, and the stage update.
     __shared__ double su[NV * kBlock];
-    const int e = element_index(prm);
-    if (e >= prm.ne) return;
+    const int lb = logical_block(prm);
+    const int base = lb * kBlock;
+    const int e = base + static_cast<int>(threadIdx.x);
+    const bool valid = e < prm.ne;
     const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
     const unsigned oe = static_cast<unsigned>(e) * 8u;
     const double* __restrict__ T = prm.tab;
     const double* __restrict__ Uin = prm.Uin;
     double* my = su + threadIdx.x;
 
-#pragma unroll
-    for (int v = 0; v < NV; ++v) my[v * kBlock] = plane_at(Uin, v, ld8, oe);
-
-    const double x0 = plane_at(prm.vtx, 0, ld8, oe), y0 = plane_at(prm.vtx, 1, ld8, oe);
-    const double x1 = plane_at(prm.vtx, 2, ld8, oe), y1 = plane_at(prm.vtx, 3, ld8, oe);
-    const double x2 = plane_at(prm.vtx, 4, ld8, oe), y2 = plane_at(prm.vtx, 5, ld8, oe);
+    // Neighbours inside this block are read from the block's LDS copy after
+    // the barrier; the others (vertical neighbours, row wrap-arounds) from
+    // global memory.
+    int loc[3];
     unsigned onb[3];
-#pragma unroll
-    for (int s = 0; s < 3; ++s) onb[s] = static_cast<unsigned>(prm.nbr[s * prm.ld + e]) * 8u;
-
-    // Edge-node values of the neighbour across side 1, in flight during the volume term.
+    double x0 = 0, y0 = 0, x1 = 0, y1 = 0, x2 = 0, y2 = 0;
     double un[NE][4];
+    if (valid) {
 #pragma unroll
-    for (int m = 0; m < NE; ++m)
+        for (int v = 0; v < NV; ++v) my[v * kBlock] = plane_at(Uin, v, ld8, oe);
+        x0 = plane_at(prm.vtx, 0, ld8, oe); y0 = plane_at(prm.vtx, 1, ld8, oe);
+        x1 = plane_at(prm.vtx, 2, ld8, oe); y1 = plane_at(prm.vtx, 3, ld8, oe);
+        x2 = plane_at(prm.vtx, 4, ld8, oe); y2 = plane_at(prm.vtx, 5, ld8, oe);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) un[m][k] = plane_at(Uin, k * NP + edge_node<P>(0, m), ld8, onb[0]);
+        for (int s = 0; s < 3; ++s) {
+            const int nb = prm.nbr[s * prm.ld + e];
+            loc[s] = nb - base;
+            onb[s] = static_cast<unsigned>(nb) * 8u;
+        }
+        // Edge-node values of the neighbour across side 1, in flight during the volume term.
+        if (static_cast<unsigned>(loc[0]) >= static_cast<unsigned>(kBlock)) {
+#pragma unroll
+            for (int m = 0; m < NE; ++m)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) un[m][k] = plane_at(Uin, k * NP + edge_node<P>(0, m), ld8, onb[0]);
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    if (static_cast<unsigned>(loc[0]) < static_cast<unsigned>(kBlock)) {
+#pragma unroll
+        for (int m = 0; m < NE; ++m)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) un[m][k] = su[(k * NP + edge_node<P>(0, m)) * kBlock + loc[0]];
+    }
 
     // Volume term: r = Sxi G^xi + Seta G^eta with G = adj(J) F (detJ * invJ = adj(J)).
     const double a00 = y2 - y0, a01 = x0 - x2, a10 = y0 - y1, a11 = x1 - x0;
@@ -183,7 +214,7 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
         const double m1 = my[(1 * NP + j) * kBlock];
         const double m2 = my[(2 * NP + j) * kBlock];
         const double m3 = my[(3 * NP + j) * kBlock];
-        const double ir = 1.0 / rho;
+        const double ir = fast_rcp(rho);
         const double uu = m1 * ir, vv = m2 * ir, E = m3 * ir;
         const double p = rho * gmi * (E - 0.5 * (uu * uu + vv * vv));
         const double H = E + p * ir;
@@ -243,11 +274,18 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
             }
         }
         if (s < 2) {
+            if (static_cast<unsigned>(loc[s + 1]) < static_cast<unsigned>(kBlock)) {
 #pragma unroll
-            for (int m = 0; m < NE; ++m)
+                for (int m = 0; m < NE; ++m)
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    un[m][k] = VT_PREFETCH_NEXT ? nxt[m][k] : plane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
+                    for (int k = 0; k < 4; ++k) un[m][k] = su[(k * NP + edge_node<P>(s + 1, m)) * kBlock + loc[s + 1]];
+            } else {
+#pragma unroll
+                for (int m = 0; m < NE; ++m)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        un[m][k] = VT_PREFETCH_NEXT ? nxt[m][k] : plane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
+            }
         }
     }
 
