@@ -1,0 +1,99 @@
+"""The C++ drop-in (vortex-transport_amd/host): the reference's call surface
+over the GPU library.
+
+CPU: the host set-up done through the drop-in classes (mesh, Element
+geometry and operators, initial state) is bit-exact with the reference's own
+translation units (golden fixtures).
+GPU: the driver with main.cpp's flow, and the reference's own main.cpp
+compiled against the drop-in, reproduce the reference's text output.
+"""
+import glob
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "vortex-transport_amd", "host")
+LIBDIR = os.path.join(ROOT, "vortex-transport_amd")
+GOLD = os.path.join(ROOT, "tests", "golden")
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+
+def _input(tmp, **kw):
+    import oracle as O
+    path = os.path.join(tmp, "input")
+    O.write_input(path, **kw)
+    return path
+
+
+@pytest.fixture(scope="module")
+def check_exe(tmp_path_factory):
+    d = tmp_path_factory.mktemp("dropin")
+    exe = str(d / "dropin_check")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I" + HOST, os.path.join(ROOT, "tests", "dropin_check.cpp"), "-o", exe,
+                    "-L" + HOST, "-lvt_dropin", "-L" + LIBDIR, "-lvt_hip", "-Wl,-rpath," + HOST, "-Wl,-rpath," + LIBDIR],
+                   check=True)
+    return exe
+
+
+@pytest.mark.parametrize("name", ["euler_p3_o8_4x4_20", "lsq_p3_o8_4x4_20", "p3_o8_4x4_perturb1_20", "p2_o8_6x5_100",
+                                  "p1_o4_7x6_100", "p0_o2_5x4_50"])
+def test_dropin_host_setup_bit_exact(check_exe, tmp_path, name):
+    d = np.load(os.path.join(GOLD, name + ".npz"))
+    p = int(d["p"])
+    kw = dict(p=p, nx=int(d["nx"]), ny=int(d["ny"]), integration_order=int(d["integration_order"]),
+              perturb_grid=int(d["perturb_grid"]), U_initialization_type=int(d["init_type"]))
+    inp = _input(str(tmp_path), **kw)
+    out = str(tmp_path / "out.bin")
+    subprocess.run([check_exe, inp, out], cwd=str(tmp_path), check=True)
+    x = np.fromfile(out)
+    ne, npn = len(d["type"]), (p + 1) * (p + 2) // 2
+    rl = 29 + 2 * npn
+    rec = x[:ne * rl].reshape(ne, rl)
+    o = ne * rl
+    U0 = x[o:o + ne * npn * 4].reshape(ne, npn, 4)
+    o += ne * npn * 4
+    F0 = x[o:o + ne * npn * 8].reshape(ne, npn, 2, 4)
+    assert np.array_equal(rec[:, 0], d["type"]) and np.array_equal(rec[:, 1:4], d["neighbors"])
+    assert np.array_equal(rec[:, 4:10], d["vertices"].reshape(ne, 6))
+    if "geometry" in d:
+        assert np.array_equal(rec[:, 10:], d["geometry"])
+    assert np.array_equal(U0, d["U0"])
+    if "F0" in d:
+        assert np.array_equal(F0, d["F0"])
+    # the drop-in's generate_mesh writes the reference's grid/ files
+    assert len(glob.glob(str(tmp_path / "grid" / "element_*.txt"))) == ne
+
+
+def _read_output(workdir, step, ne):
+    vals = []
+    for e in range(ne):
+        rows = np.loadtxt(os.path.join(workdir, "output", "step_%d" % step, "element_%d.txt" % e), skiprows=1)
+        vals.append(rows)
+    return np.array(vals)  # [ne][Np][16]: node time x y u0..3 fx0..3 fy0..3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("exe", ["vortex_transport_gpu", "vt_main_on_gpu"])
+def test_drivers_reproduce_reference_output(gpu, tmp_path, exe):
+    path = os.path.join(HOST, exe) if exe == "vortex_transport_gpu" else os.path.join(ROOT, "oracle", "_ref", exe)
+    if not os.path.exists(path):
+        pytest.skip("%s not built" % exe)
+    # the C1 input: dt = 14.14213562373 / 10000 as in the fixtures, a dump every 250 steps
+    d250 = np.load(os.path.join(GOLD, "c1_p3_o8_8x8_250.npz"))
+    dfull = np.load(os.path.join(GOLD, "c1_p3_o8_8x8_full.npz"))
+    inp = _input(str(tmp_path), p=3, nx=8, ny=8, integration_order=8, number_time_steps=10000, write_every_steps=250)
+    r = subprocess.run([path, inp], cwd=str(tmp_path), capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "[TIMING] timestepping_time_sec" in r.stdout
+    ne = len(d250["type"])
+    for step, d in ((0, None), (250, d250), (10000, dfull)):
+        out = _read_output(str(tmp_path), step, ne)
+        want = d250["U0"] if d is None else d["U"]
+        # the dumps hold 6-decimal text of the values (std::to_string)
+        assert np.abs(out[:, :, 4:8] - want).max() <= 5.01e-7, step
+        if d is not None:
+            assert np.allclose(out[:, :, 1], float("%f" % d["time"]))

@@ -4,7 +4,10 @@
 
 namespace vt {
 
-constexpr int kBlock = 256;
+#ifndef VT_BLOCK
+#define VT_BLOCK 256
+#endif
+constexpr int kBlock = VT_BLOCK;
 
 enum StageMode : int {
     kModeRhs = 0,    // Uout = dU/dt                         (evolve_elem)

@@ -1,11 +1,9 @@
 // Drop-in declaration of the element class (Element.H:10-53): the same public
-// members and methods.  The reference operators handed to
-// build_mass_matrix_inverse / build_stiffness_matrix are also kept (shared
-// between elements) so the GPU context can be built from the elements alone.
+// members, in the same order and types (so code built against either header
+// sees one object layout), and the same methods.
 #ifndef ELEMENT_H
 #define ELEMENT_H
 
-#include <memory>
 #include <string>
 #include <vector>
 
@@ -46,10 +44,6 @@ public:
     void build_mass_matrix_inverse(const std::vector<std::vector<double>>& inv_mass_matrix);
     void build_stiffness_matrix(const std::vector<std::vector<std::vector<double>>>& stiff_matrix);
     void build_jacobians();
-
-    // Reference operators as passed to the two build_* methods ([Np][Np] and [2][Np][Np], flat).
-    std::shared_ptr<const std::vector<double>> reference_inverse_mass;
-    std::shared_ptr<const std::vector<double>> reference_stiffness;
 };
 
 #endif  // ELEMENT_H
