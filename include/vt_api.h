@@ -133,6 +133,50 @@ int vt_stage_kernel_times(vt_ctx *ctx, double dt, int nsteps, double *ms_per_sta
  * the reference's error number. */
 int vt_numerical_flux(const double *records, double *flux /* [n][4] */, int *codes, int n);
 
+/* Launch shape of the stage kernels: block = elements per workgroup (128 or
+ * 256); persistent != 0 runs 8 * slots_per_xcd resident workgroups that loop
+ * over the element tiles instead of one workgroup per tile.  Results do not
+ * depend on it.  No reference counterpart (the reference's OpenMP loop,
+ * Evolve.cpp:7-9, has no launch shape). */
+int vt_set_launch(vt_ctx *ctx, int block, int persistent, int slots_per_xcd);
+int vt_get_launch(const vt_ctx *ctx, int *cfg /* [3] block, persistent, slots_per_xcd */);
+
+/* ---- ensemble ------------------------------------------------------------ */
+/* Many independent problems advanced together: no reference counterpart (the
+ * reference runs one mesh per process, main.cpp:38-176).  Each instance is
+ * what vt_create would build from its own arrays; every RK stage of all
+ * instances is one launch per quadrature size.  Instances share dt, step count
+ * and time. */
+typedef struct vt_batch vt_batch;
+
+/* n_elem[i], p[i], order[i] per instance; vertices [sum n][3][2], neighbors
+ * [sum n][3] (indices local to the instance) and type [sum n] concatenated in
+ * instance order. */
+int vt_batch_create(int n_inst, const int *n_elem, const int *p, const int *order, const double *vertices,
+                    const int *neighbors, const int *type, vt_batch **out);
+void vt_batch_destroy(vt_batch *b);
+int vt_batch_num_instances(const vt_batch *b);
+/* Kernel launches per RK stage (distinct quadrature sizes, at most 4 orders p each). */
+int vt_batch_num_launches(const vt_batch *b);
+/* Vortex initial state per instance (vt_init_vortex with epsilon[i], x0[i], y0[i]). */
+int vt_batch_init_vortex(vt_batch *b, int init_type, const double *epsilon, const double *x0, const double *y0);
+int vt_batch_set_state(vt_batch *b, int inst, const double *U /* [n_elem[inst]][Np][4] */);
+int vt_batch_get_state(vt_batch *b, int inst, double *U);
+double vt_batch_get_time(const vt_batch *b);
+int vt_batch_set_time(vt_batch *b, double t);
+/* vt_step over all instances; VT_ERR_NONPHYSICAL at the first failing flux
+ * evaluation of any instance, located by vt_batch_last_error. */
+int vt_batch_step(vt_batch *b, int method, double dt, int nsteps);
+int vt_batch_step_timed(vt_batch *b, int method, double dt, int nsteps, double *ms);
+/* instance, element (local), quadrature point, side, code, step, stage. */
+int vt_batch_last_error(const vt_batch *b, int *info /* [7] */);
+int vt_batch_set_launch(vt_batch *b, int block, int persistent, int slots_per_xcd);
+int vt_batch_get_launch(const vt_batch *b, int *cfg /* [3] */);
+/* Times trial_steps (1..64) RK4 steps of dt for each candidate launch shape,
+ * from the current state each time, keeps the fastest and restores state
+ * and time.  best_cfg [3] and best_ms (per step) may be NULL. */
+int vt_batch_autotune(vt_batch *b, double dt, int trial_steps, int *best_cfg, double *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,201 @@
+"""Ensemble path (vt_batch_*): every instance must equal the plain
+single-mesh run of the same instance, which the other GPU tests pin to the
+oracle (relative L2 <= 1e-12, checked here too for some instances).
+
+Batched, single-mesh and other launch shapes run the same per-element source
+in different kernels; the compiler's FMA contraction may differ between them
+at the last bit, so "equal" is relative L2 <= 1e-14 (observed: 0 to ~5e-17
+after 70 steps).  Set-up data (initial state, set/get round trips) is
+compared bit-exactly.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+# Mixed sizes, orders p and two integration orders (two quadrature sizes ->
+# two launches per stage), ragged tile counts (not multiples of 8 tiles).
+SMALL = [  # cells, p, order, epsilon, x0, y0
+    (4, 3, 8, 0.3, 0.0, 0.0),
+    (7, 1, 8, 0.2, 1.0, -0.5),
+    (12, 2, 8, 0.35, -1.5, 2.0),
+    (5, 0, 8, 0.1, 0.3, 0.3),
+    (9, 3, 8, 0.25, 2.0, 1.0),
+    (6, 3, 6, 0.4, -2.0, -2.0),
+    (3, 2, 6, 0.15, 0.5, -1.0),
+    (33, 3, 8, 0.3, 0.0, 1.0),
+]
+
+
+SAME = 1e-14
+
+
+def same(a, b):
+    return rel(a, b) <= SAME
+
+
+def _build(vt, cfg):
+    meshes = [vt.generate_mesh(nx=c[0], ny=c[0]) for c in cfg]
+    b = vt.Batch(meshes, [c[1] for c in cfg], [c[2] for c in cfg])
+    b.initialize_hydrodinamics(0, [c[3] for c in cfg], [c[4] for c in cfg], [c[5] for c in cfg])
+    return meshes, b
+
+
+def _single(vt, mesh, c, method, dt, steps):
+    s = vt.Solver(mesh, c[1], c[2])
+    s.initialize_hydrodinamics(0, c[3], c[4], c[5])
+    s.step(method, dt, steps)
+    U = s.state()
+    s.close()
+    return U
+
+
+def test_batch_launch_count(vt, gpu):
+    _, b = _build(vt, SMALL)
+    assert b.num_launches == 2
+    assert b.n == len(SMALL)
+
+
+def test_batch_initial_state_equals_single(vt, gpu):
+    meshes, b = _build(vt, SMALL)
+    for i, (m, c) in enumerate(zip(meshes, SMALL)):
+        s = vt.Solver(m, c[1], c[2])
+        s.initialize_hydrodinamics(0, c[3], c[4], c[5])
+        assert np.array_equal(b.state(i), s.state())
+
+
+@pytest.mark.parametrize("method,steps", [(1, 70), (0, 67)])
+def test_batch_equals_single_instances(vt, gpu, method, steps):
+    dt = 1e-3
+    meshes, b = _build(vt, SMALL)
+    b.step(method, dt, steps)
+    assert abs(b.time - steps * dt) < 1e-12
+    for i, (m, c) in enumerate(zip(meshes, SMALL)):
+        U = _single(vt, m, c, method, dt, steps)
+        assert same(b.state(i), U), (i, rel(b.state(i), U))
+
+
+def test_batch_instances_match_oracle(vt, oracle, gpu):
+    dt, steps = 1e-3, 40
+    meshes, b = _build(vt, SMALL)
+    b.rk4(dt, steps)
+    for i in (0, 1, 2, 5):
+        m, c = meshes[i], SMALL[i]
+        o = oracle.Solver(m.element_coordinates, m.elements_at_boundary, m.element_type, c[1], c[2])
+        o.init_vortex(0, c[3], c[4], c[5])
+        assert o.run(1, dt, steps) == 0
+        assert rel(b.state(i), o.state()) < 1e-12
+
+
+@pytest.mark.parametrize("block,persistent,slots", [(256, False, 0), (128, False, 0), (256, True, 1), (128, True, 3)])
+def test_batch_launch_shapes_agree(vt, gpu, block, persistent, slots):
+    dt, steps = 1e-3, 9
+    meshes, b = _build(vt, SMALL)
+    b.set_launch(block, persistent, slots)
+    assert b.launch() == {"block": block, "persistent": persistent, "slots_per_xcd": slots}
+    b.rk4(dt, steps)
+    for i, (m, c) in enumerate(zip(meshes, SMALL)):
+        assert same(b.state(i), _single(vt, m, c, 1, dt, steps)), i
+
+
+@pytest.mark.parametrize("block,persistent,slots", [(128, False, 0), (256, True, 2), (128, True, 5)])
+def test_single_mesh_launch_shapes_agree(vt, gpu, block, persistent, slots):
+    mesh = vt.generate_mesh(nx=40, ny=40)
+    a = vt.Solver(mesh, 3, 8)
+    a.initialize_hydrodinamics(0)
+    a.rk4(1e-3, 6)
+    s = vt.Solver(mesh, 3, 8)
+    s.set_launch(block, persistent, slots)
+    s.initialize_hydrodinamics(0)
+    s.rk4(1e-3, 6)
+    assert same(s.state(), a.state())
+
+
+def test_batch_autotune_keeps_state_and_time(vt, gpu):
+    meshes, b = _build(vt, SMALL)
+    b.rk4(1e-3, 3)
+    U = [b.state(i) for i in range(b.n)]
+    t = b.time
+    cfg, ms = b.autotune(1e-3, trial_steps=2)
+    assert cfg["block"] in (128, 256) and ms > 0
+    assert b.launch() == cfg
+    assert b.time == t
+    for i in range(b.n):
+        assert np.array_equal(b.state(i), U[i])
+
+
+def test_batch_set_state_roundtrip(vt, gpu):
+    meshes, b = _build(vt, SMALL)
+    rng = np.random.default_rng(5)
+    X = [rng.standard_normal(b.state(i).shape) for i in range(b.n)]
+    for i in reversed(range(b.n)):
+        b.set_state(i, X[i])
+    for i in range(b.n):
+        assert np.array_equal(b.state(i), X[i])
+
+
+def test_batch_nonphysical_names_the_instance(vt, oracle, gpu):
+    meshes, b = _build(vt, SMALL)
+    k = 4
+    U = b.state(k)
+    U[11, 3, 0] = -1.0
+    b.set_state(k, U)
+    with pytest.raises(vt.NonPhysicalStateError) as e:
+        b.rk4(1e-3, 5)
+    assert e.value.instance == k
+    m, c = meshes[k], SMALL[k]
+    o = oracle.Solver(m.element_coordinates, m.elements_at_boundary, m.element_type, c[1], c[2])
+    o.set_state(U)
+    assert o.run(1, 1e-3, 5) != 0
+    assert tuple(e.value.info) == o.last_error()
+
+
+def test_batch_reports_the_reference_failure_of_an_under_integrated_instance(vt, oracle, gpu):
+    # p = 3 with integration_order 4: the reference's mass matrix is singular
+    # (condition ~1e17) and its run exits in step 1, stage 2.  Inside an
+    # ensemble the failure must be named with the same location.
+    cfg = SMALL[:3] + [(6, 3, 4, 0.4, -2.0, -2.0)] + SMALL[3:5]
+    meshes, b = _build(vt, cfg)
+    with pytest.raises(vt.NonPhysicalStateError) as e:
+        b.rk4(1e-3, 10)
+    assert e.value.instance == 3
+    m, c = meshes[3], cfg[3]
+    o = oracle.Solver(m.element_coordinates, m.elements_at_boundary, m.element_type, c[1], c[2])
+    o.init_vortex(0, c[3], c[4], c[5])
+    assert o.run(1, 1e-3, 10) != 0
+    assert tuple(e.value.info) == o.last_error()
+
+
+def test_batch_rejects_bad_instances(vt, gpu):
+    m = vt.generate_mesh(nx=4, ny=4)
+    with pytest.raises(vt.VortexError) as e:
+        vt.Batch([m, m], [3, 4], 8)
+    assert e.value.status == vt.VT_ERR_UNSUPPORTED_P
+    with pytest.raises(vt.VortexError) as e:
+        vt.Batch([m], 3, 30)
+    assert e.value.status == vt.VT_ERR_QUAD
+    _, b = _build(vt, SMALL[:2])
+    for i in (2, -1):
+        with pytest.raises(IndexError):
+            b.state(i)
+
+
+# --- the 64-instance ensemble: cells 32..256, p 1..3, seed 3 -------------------
+
+def test_ensemble_64_equals_single_instances(vt, gpu):
+    draws = vt.ensemble_draws(3, 64)
+    meshes = [vt.generate_mesh(nx=d["cells"], ny=d["cells"]) for d in draws]
+    b = vt.Batch(meshes, [d["p"] for d in draws], 8)
+    b.initialize_hydrodinamics(0, [d["epsilon"] for d in draws], [d["x0"] for d in draws], [d["y0"] for d in draws])
+    assert b.num_launches == 1
+    dt, steps = 1e-4, 5
+    b.rk4(dt, steps)
+    for i in range(0, 64, 7):
+        d = draws[i]
+        U = _single(vt, meshes[i], (d["cells"], d["p"], 8, d["epsilon"], d["x0"], d["y0"]), 1, dt, steps)
+        assert same(b.state(i), U), i

@@ -121,3 +121,36 @@ def test_read_input_files_matches_reference_parsing(vt, tmp_path):
     f.write_text(src.replace("p=3", "p = 3"))
     with pytest.raises(ValueError):          # key "p " does not match "p": stoi("") throws
         vt.read_input_files(str(f))
+
+
+def test_splitmix64_known_answers(vt):
+    # Published splitmix64 outputs for seed 0 (Vigna's reference generator).
+    g = vt.splitmix64(0)
+    assert [next(g) for _ in range(3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_ensemble_draws_ranges_and_determinism(vt):
+    a = vt.ensemble_draws(3, 64)
+    assert a == vt.ensemble_draws(3, 64)
+    assert a != vt.ensemble_draws(4, 64)
+    assert all(32 <= d["cells"] <= 256 and d["p"] in (1, 2, 3) for d in a)
+    assert all(0.1 <= d["epsilon"] < 0.4 and -2.5 <= d["x0"] < 2.5 and -2.5 <= d["y0"] < 2.5 for d in a)
+    # all three orders and a spread of sizes occur in the 64 draws
+    assert {d["p"] for d in a} == {1, 2, 3}
+    assert max(d["cells"] for d in a) - min(d["cells"] for d in a) > 150
+
+
+def test_batch_create_rejects_bad_arguments_before_touching_the_device(vt):
+    m = vt.generate_mesh(nx=4, ny=4)
+    for p, order, status in [([3, 4], 8, vt.VT_ERR_UNSUPPORTED_P), ([3, 3], 25, vt.VT_ERR_QUAD)]:
+        with pytest.raises(vt.VortexError) as e:
+            vt.Batch([m, m], p, order)
+        assert e.value.status == status
+    bad = vt.Mesh(m.element_coordinates, m.elements_at_boundary.copy(), m.element_type)
+    bad.elements_at_boundary[3, 1] = m.num_elements  # local indices: one past the instance
+    with pytest.raises(vt.VortexError) as e:
+        vt.Batch([m, bad], 3, 8)
+    assert e.value.status == vt.VT_ERR_ARG
+    with pytest.raises(vt.VortexError) as e:
+        vt.Batch([], 3, 8)
+    assert e.value.status == vt.VT_ERR_ARG

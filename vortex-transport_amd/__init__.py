@@ -12,6 +12,8 @@ include/vt_api.h (libvt_hip.so, built in-tree by build.py):
     .rk4(dt) / .forward_euler(dt)  Timestepping.cpp:85-305 / 39-82
     .step(method, dt, nsteps)   the time loop of main.cpp:154-170
   numerical_flux(UL, UR, n)     Numericalflux.cpp:13-156 (device)
+  Batch(meshes, p, order)       many independent instances, one launch per RK
+                                stage (no reference counterpart)
 
 Non-physical states raise NonPhysicalStateError where the reference prints
 "N. Error: Non-physical state!" and exits (Numericalflux.cpp:51-124).  The
@@ -44,14 +46,20 @@ class VortexError(RuntimeError):
 
 
 class NonPhysicalStateError(VortexError):
-    """Raised where the reference exits on rho <= 0, p <= 0 or c2 <= 0."""
+    """Raised where the reference exits on rho <= 0, p <= 0 or c2 <= 0.
 
-    def __init__(self, info):
+    info = (element, quadrature point, side, code, step, stage); for an
+    ensemble, `instance` names the instance the element belongs to."""
+
+    def __init__(self, info, instance=None):
         elem, q, side, code, step, stage = info
         msg = "%d. Error: Non-physical state! (element %d, side %d, quadrature point %d, step %d, stage %d)" % (
             code, elem, side + 1, q, step, stage)
+        if instance is not None:
+            msg += " in instance %d" % instance
         super().__init__(VT_ERR_NONPHYSICAL, msg)
         self.info = info
+        self.instance = instance
 
 
 def lib():
@@ -88,6 +96,23 @@ def lib():
             "vt_step_timed": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
             "vt_stage_kernel_times": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, _dp]),
             "vt_numerical_flux": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int]),
+            "vt_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "vt_get_launch": (ctypes.c_int, [vp, _ip]),
+            "vt_batch_create": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip, _dp, _ip, _ip, ctypes.POINTER(vp)]),
+            "vt_batch_destroy": (None, [vp]),
+            "vt_batch_num_instances": (ctypes.c_int, [vp]),
+            "vt_batch_num_launches": (ctypes.c_int, [vp]),
+            "vt_batch_init_vortex": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp]),
+            "vt_batch_set_state": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
+            "vt_batch_get_state": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
+            "vt_batch_get_time": (ctypes.c_double, [vp]),
+            "vt_batch_set_time": (ctypes.c_int, [vp, ctypes.c_double]),
+            "vt_batch_step": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+            "vt_batch_step_timed": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
+            "vt_batch_last_error": (ctypes.c_int, [vp, _ip]),
+            "vt_batch_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "vt_batch_get_launch": (ctypes.c_int, [vp, _ip]),
+            "vt_batch_autotune": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -108,6 +133,10 @@ def _i(a):
 def _check(rc, solver=None):
     if rc == VT_OK:
         return
+    if rc == VT_ERR_NONPHYSICAL and isinstance(solver, Batch):
+        info = np.zeros(7, dtype=np.int32)
+        lib().vt_batch_last_error(solver._h, _i(info))
+        raise NonPhysicalStateError(tuple(int(x) for x in info[1:]), instance=int(info[0]))
     if rc == VT_ERR_NONPHYSICAL and solver is not None:
         info = np.zeros(6, dtype=np.int32)
         lib().vt_last_error(solver._h, _i(info))
@@ -352,3 +381,154 @@ class Solver:
         ms = np.zeros(4)
         _check(lib().vt_stage_kernel_times(self._h, dt, nsteps, _d(ms)), self)
         return ms
+
+    def set_launch(self, block=256, persistent=False, slots_per_xcd=0):
+        _check(lib().vt_set_launch(self._h, block, int(bool(persistent)), slots_per_xcd))
+
+    def launch(self):
+        c = np.zeros(3, dtype=np.int32)
+        _check(lib().vt_get_launch(self._h, _i(c)))
+        return {"block": int(c[0]), "persistent": bool(c[1]), "slots_per_xcd": int(c[2])}
+
+
+# ---------------------------------------------------------------------------
+# Ensemble: many independent meshes in one launch per RK stage (no reference
+# counterpart; each instance is what Solver builds from its own mesh)
+# ---------------------------------------------------------------------------
+
+class Batch:
+    """Device-resident ensemble of independent vortex problems.
+
+    meshes: list of Mesh; p, integration_order: one int each or one per
+    instance.  Instances share dt, step count and time."""
+
+    def __init__(self, meshes, p, integration_order):
+        n = len(meshes)
+        if n < 1:
+            raise VortexError(VT_ERR_ARG, "an ensemble needs at least one instance")
+        ps = [p] * n if np.isscalar(p) else list(p)
+        orders = [integration_order] * n if np.isscalar(integration_order) else list(integration_order)
+        if len(ps) != n or len(orders) != n:
+            raise ValueError("p and integration_order need one entry per instance")
+        self.n = n
+        self.p = [int(x) for x in ps]
+        self.order = [int(x) for x in orders]
+        self.ne = [m.num_elements for m in meshes]
+        self.np = [num_nodes(x) for x in self.p]
+        self._ne = np.array(self.ne, dtype=np.int32)
+        self._p = np.array(self.p, dtype=np.int32)
+        self._o = np.array(self.order, dtype=np.int32)
+        v = np.ascontiguousarray(np.concatenate([m.element_coordinates.reshape(-1) for m in meshes]), dtype=np.float64)
+        b = np.ascontiguousarray(np.concatenate([m.elements_at_boundary.reshape(-1) for m in meshes]), dtype=np.int32)
+        t = np.ascontiguousarray(np.concatenate([m.element_type.reshape(-1) for m in meshes]), dtype=np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().vt_batch_create(n, _i(self._ne), _i(self._p), _i(self._o), _d(v), _i(b), _i(t), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.vt_batch_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def num_launches(self):
+        """Kernel launches per RK stage."""
+        return lib().vt_batch_num_launches(self._h)
+
+    def initialize_hydrodinamics(self, init_type, epsilon, x0, y0):
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (self.n,)))
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(x0, dtype=np.float64), (self.n,)))
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(y0, dtype=np.float64), (self.n,)))
+        _check(lib().vt_batch_init_vortex(self._h, init_type, _d(e), _d(a), _d(c)), self)
+
+    def _inst(self, i):
+        if not 0 <= i < self.n:
+            raise IndexError("instance %d out of range (ensemble of %d)" % (i, self.n))
+        return i
+
+    def set_state(self, i, U):
+        i = self._inst(i)
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.ne[i], self.np[i], 4)
+        _check(lib().vt_batch_set_state(self._h, i, _d(U)))
+
+    def state(self, i):
+        i = self._inst(i)
+        U = np.zeros((self.ne[i], self.np[i], 4))
+        _check(lib().vt_batch_get_state(self._h, i, _d(U)))
+        return U
+
+    @property
+    def time(self):
+        return lib().vt_batch_get_time(self._h)
+
+    @time.setter
+    def time(self, t):
+        _check(lib().vt_batch_set_time(self._h, t))
+
+    def step(self, method, dt, nsteps=1):
+        _check(lib().vt_batch_step(self._h, method, dt, nsteps), self)
+
+    def rk4(self, dt, nsteps=1):
+        self.step(1, dt, nsteps)
+
+    def forward_euler(self, dt, nsteps=1):
+        self.step(0, dt, nsteps)
+
+    def step_timed(self, method, dt, nsteps):
+        ms = ctypes.c_double()
+        _check(lib().vt_batch_step_timed(self._h, method, dt, nsteps, ctypes.byref(ms)), self)
+        return ms.value
+
+    def set_launch(self, block=256, persistent=False, slots_per_xcd=0):
+        _check(lib().vt_batch_set_launch(self._h, block, int(bool(persistent)), slots_per_xcd))
+
+    def launch(self):
+        c = np.zeros(3, dtype=np.int32)
+        _check(lib().vt_batch_get_launch(self._h, _i(c)))
+        return {"block": int(c[0]), "persistent": bool(c[1]), "slots_per_xcd": int(c[2])}
+
+    def autotune(self, dt, trial_steps=4):
+        """Picks the fastest launch shape over trial RK4 steps; state and
+        time are left unchanged.  Returns (launch dict, ms per step)."""
+        c = np.zeros(3, dtype=np.int32)
+        ms = ctypes.c_double()
+        _check(lib().vt_batch_autotune(self._h, dt, trial_steps, _i(c), ctypes.byref(ms)), self)
+        return {"block": int(c[0]), "persistent": bool(c[1]), "slots_per_xcd": int(c[2])}, ms.value
+
+
+def splitmix64(seed):
+    """splitmix64 stream: yields uint64 values (Python ints)."""
+    mask = (1 << 64) - 1
+    x = seed & mask
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & mask
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        yield z ^ (z >> 31)
+
+
+def ensemble_draws(seed, n_inst, cells=(32, 256), orders_p=(1, 2, 3), eps=(0.1, 0.4), centre=(-2.5, 2.5)):
+    """Per-instance draws of an ensemble: cells per side uniform in
+    cells[0]..cells[1], p uniform over orders_p, epsilon uniform in eps and
+    the centre uniform in centre^2.  Uniform doubles are (x >> 11) * 2^-53 of
+    a splitmix64 stream; integers are lo + floor(u * count).  Draw order per
+    instance: cells, p, epsilon, x0, y0."""
+    g = splitmix64(seed)
+
+    def u():
+        return (next(g) >> 11) * (1.0 / (1 << 53))
+
+    out = []
+    for _ in range(n_inst):
+        n = cells[0] + int(u() * (cells[1] - cells[0] + 1))
+        p = orders_p[int(u() * len(orders_p))]
+        e = eps[0] + (eps[1] - eps[0]) * u()
+        x0 = centre[0] + (centre[1] - centre[0]) * u()
+        y0 = centre[0] + (centre[1] - centre[0]) * u()
+        out.append({"cells": n, "p": p, "epsilon": e, "x0": x0, "y0": y0})
+    return out

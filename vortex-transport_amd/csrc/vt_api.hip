@@ -4,16 +4,12 @@
 
 #include <cstdio>
 #include <cstring>
-#include <map>
-#include <tuple>
 #include <vector>
 
-#include "vt_api.h"
+#include "vt_ctx.hpp"
 #include "vt_device.hpp"
-#include "vt_kernels.hpp"
-#include "vt_setup.hpp"
 
-namespace {
+namespace vti {
 
 thread_local char g_hip_error[256] = "";
 
@@ -22,34 +18,48 @@ bool hip_ok(hipError_t e, const char* what) {
     std::snprintf(g_hip_error, sizeof(g_hip_error), "%s: %s", what, hipGetErrorString(e));
     return false;
 }
-#define HIP_TRY(expr)                                   \
-    do {                                                \
-        if (!hip_ok((expr), #expr)) return VT_ERR_HIP;  \
-    } while (0)
 
-// Plane stride: element count rounded to 256 plus a 64-element skew, so the
-// 4*Np planes of one element do not all start at the same HBM channel offset.
-int plane_stride(int ne) { return ((ne + 255) / 256) * 256 + 64; }
+void to_planes(const vt_ctx* c, const double* U, int e0, int n, double* planes) {
+    for (int e = 0; e < n; ++e)
+        for (int j = 0; j < c->np; ++j)
+            for (int k = 0; k < 4; ++k)
+                planes[static_cast<size_t>(k * c->np + j) * c->ld + e0 + e] = U[(static_cast<size_t>(e) * c->np + j) * 4 + k];
+}
 
-// Graph steps per replay: one graph covers up to kGraphSteps steps.
-constexpr int kGraphSteps = 64;
+void from_planes(const vt_ctx* c, const double* planes, int e0, int n, double* U) {
+    for (int e = 0; e < n; ++e)
+        for (int j = 0; j < c->np; ++j)
+            for (int k = 0; k < 4; ++k)
+                U[(static_cast<size_t>(e) * c->np + j) * 4 + k] = planes[static_cast<size_t>(k * c->np + j) * c->ld + e0 + e];
+}
 
-}  // namespace
+std::vector<int> replay_plan(int method, int nsteps) {
+    std::vector<int> reps;
+    int left = nsteps;
+    while (left > 0) {
+        int k = left < kGraphSteps ? left : kGraphSteps;
+        if (method == 0 && k > 1 && (k % 2)) k -= 1;
+        reps.push_back(k);
+        left -= k;
+    }
+    return reps;
+}
 
-struct vt_ctx {
-    int ne = 0, ld = 0, p = 0, np = 0, nq = 0, order = 0, device = 0;
-    hipStream_t stream = nullptr;
-    double *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr, *vtx = nullptr, *tab = nullptr;
-    int* nbr = nullptr;
-    unsigned long long* err = nullptr;       // device error key
-    unsigned long long* err_init = nullptr;  // pinned host, holds kNoError
-    unsigned long long* err_host = nullptr;  // pinned host, one slot per replay
-    size_t err_host_cap = 0;
-    std::vector<double> vertices;
-    double time = 0.0;
-    std::map<std::tuple<int, unsigned long long, int>, hipGraphExec_t> graphs;
-    int last_err[6] = {-1, -1, -1, 0, 0, 0};
-};
+int ensure_err_slots(unsigned long long** slots, size_t* cap, size_t n) {
+    if (n <= *cap) return VT_OK;
+    if (*slots) hipHostFree(*slots);
+    *slots = nullptr;
+    *cap = 0;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(slots), n * sizeof(unsigned long long), hipHostMallocDefault));
+    *cap = n;
+    return VT_OK;
+}
+
+}  // namespace vti
+
+using vti::hip_ok;
+using vti::plane_stride;
+using vti::kGraphSteps;
 
 namespace {
 
@@ -61,7 +71,7 @@ vt::StageParams base_params(vt_ctx* c) {
     prm.err = c->err;
     prm.ne = c->ne;
     prm.ld = c->ld;
-    prm.blocks_per_xcd = vt::blocks_per_xcd(c->ne);
+    prm.blocks_per_xcd = vt::tiles_per_xcd(c->ne, c->cfg.block);
     return prm;
 }
 
@@ -81,7 +91,7 @@ hipError_t enqueue_rk4_step(vt_ctx* c, double dt, int stage_base) {
         prm.acc = c->acc;
         prm.mode = st[s].mode;
         prm.stage_id = stage_base + s;
-        hipError_t e = vt::launch_stage(prm, c->p, c->nq, c->stream);
+        hipError_t e = vt::launch_stage(prm, c->p, c->nq, c->cfg, c->stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -94,7 +104,7 @@ hipError_t enqueue_euler_step(vt_ctx* c, double dt, const double* src, double* d
     prm.Uout = dst;
     prm.mode = vt::kModeEuler;
     prm.stage_id = stage_id;
-    return vt::launch_stage(prm, c->p, c->nq, c->stream);
+    return vt::launch_stage(prm, c->p, c->nq, c->cfg, c->stream);
 }
 
 // Graph of k steps (Euler graphs hold an even number of steps, alternating
@@ -123,26 +133,11 @@ int get_graph(vt_ctx* c, int method, double dt, int k, hipGraphExec_t* out) {
     return VT_OK;
 }
 
-void to_planes(const vt_ctx* c, const double* U, std::vector<double>* planes) {
-    planes->assign(static_cast<size_t>(4) * c->np * c->ld, 0.0);
-    for (int e = 0; e < c->ne; ++e)
-        for (int j = 0; j < c->np; ++j)
-            for (int k = 0; k < 4; ++k)
-                (*planes)[static_cast<size_t>(k * c->np + j) * c->ld + e] = U[(static_cast<size_t>(e) * c->np + j) * 4 + k];
-}
-
-void from_planes(const vt_ctx* c, const std::vector<double>& planes, double* U) {
-    for (int e = 0; e < c->ne; ++e)
-        for (int j = 0; j < c->np; ++j)
-            for (int k = 0; k < 4; ++k)
-                U[(static_cast<size_t>(e) * c->np + j) * 4 + k] = planes[static_cast<size_t>(k * c->np + j) * c->ld + e];
-}
-
 int read_planes(vt_ctx* c, const double* dev, double* U) {
     std::vector<double> h(static_cast<size_t>(4) * c->np * c->ld);
     HIP_TRY(hipMemcpyAsync(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    from_planes(c, h, U);
+    vti::from_planes(c, h.data(), 0, c->ne, U);
     return VT_OK;
 }
 
@@ -157,27 +152,11 @@ void decode_error(vt_ctx* c, unsigned long long key, int step_base, int per_step
     c->last_err[5] = static_cast<int>(stage % per_step) + 1;
 }
 
-int ensure_err_slots(vt_ctx* c, size_t n) {
-    if (n <= c->err_host_cap) return VT_OK;
-    if (c->err_host) hipHostFree(c->err_host);
-    c->err_host = nullptr;
-    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->err_host), n * sizeof(unsigned long long), hipHostMallocDefault));
-    c->err_host_cap = n;
-    return VT_OK;
-}
-
 int run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0, hipEvent_t ev1) {
     if (method != 0 && method != 1) return VT_ERR_METHOD;
     if (nsteps < 0) return VT_ERR_ARG;
-    std::vector<int> reps;
-    int left = nsteps;
-    while (left > 0) {
-        int k = left < kGraphSteps ? left : kGraphSteps;
-        if (method == 0 && k > 1 && (k % 2)) k -= 1;
-        reps.push_back(k);
-        left -= k;
-    }
-    int rc = ensure_err_slots(c, reps.size() + 1);
+    const std::vector<int> reps = vti::replay_plan(method, nsteps);
+    int rc = vti::ensure_err_slots(&c->err_host, &c->err_host_cap, reps.size() + 1);
     if (rc) return rc;
     std::vector<hipGraphExec_t> ex(reps.size(), nullptr);
     for (size_t r = 0; r < reps.size(); ++r)
@@ -230,7 +209,7 @@ const char* vt_status_string(int s) {
     }
 }
 
-const char* vt_last_hip_error(void) { return g_hip_error; }
+const char* vt_last_hip_error(void) { return vti::g_hip_error; }
 
 int vt_mesh_generate(int nx, int ny, float dx, float dy, float perturb, double* v, int* nb, int* t) {
     if (!v || !nb || !t) return VT_ERR_ARG;
@@ -264,7 +243,9 @@ int vt_quadrature(int order, int* nq_line, double* line_xw, int* n_area, double*
     return VT_OK;
 }
 
-namespace {
+}  // extern "C"
+
+namespace vti {
 
 int create_ctx(int n, const double* vertices, const int* neighbors, const int* type, int p, const vt::OperatorTable& tab,
                vt_ctx** out) {
@@ -309,7 +290,9 @@ int create_ctx(int n, const double* vertices, const int* neighbors, const int* t
     return VT_OK;
 }
 
-}  // namespace
+}  // namespace vti
+
+extern "C" {
 
 int vt_create(int n, const double* vertices, const int* neighbors, const int* type, int p, int order, vt_ctx** out) {
     if (!out) return VT_ERR_ARG;
@@ -318,7 +301,7 @@ int vt_create(int n, const double* vertices, const int* neighbors, const int* ty
     if (p < 0 || p > vt::kMaxP) return VT_ERR_UNSUPPORTED_P;
     vt::OperatorTable tab;
     if (vt::line_rule(order, nullptr) < 0 || !vt::build_operator_table(p, order, &tab)) return VT_ERR_QUAD;
-    return create_ctx(n, vertices, neighbors, type, p, tab, out);
+    return vti::create_ctx(n, vertices, neighbors, type, p, tab, out);
 }
 
 int vt_create_with_operators(int n, const double* vertices, const int* neighbors, const int* type, int p, int nq,
@@ -331,7 +314,7 @@ int vt_create_with_operators(int n, const double* vertices, const int* neighbors
     vt::OperatorTable tab;
     if (!vt::build_operator_table(p, nq, line_xw, Minv_ref, S_ref, &tab)) return VT_ERR_ARG;
     tab.order = -1;
-    return create_ctx(n, vertices, neighbors, type, p, tab, out);
+    return vti::create_ctx(n, vertices, neighbors, type, p, tab, out);
 }
 
 void vt_destroy(vt_ctx* c) {
@@ -367,8 +350,8 @@ int vt_exact_vortex(const vt_ctx* c, double t, double epsilon, double x0, double
 
 int vt_set_state(vt_ctx* c, const double* U) {
     if (!c || !U) return VT_ERR_ARG;
-    std::vector<double> h;
-    to_planes(c, U, &h);
+    std::vector<double> h(static_cast<size_t>(4) * c->np * c->ld, 0.0);
+    vti::to_planes(c, U, 0, c->ne, h.data());
     HIP_TRY(hipMemcpyAsync(c->Un, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return VT_OK;
@@ -403,7 +386,7 @@ int vt_rhs(vt_ctx* c, double* dUdt) {
     prm.mode = vt::kModeRhs;
     prm.stage_id = 0;
     HIP_TRY(hipMemcpyAsync(c->err, c->err_init, sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(vt::launch_stage(prm, c->p, c->nq, c->stream));
+    HIP_TRY(vt::launch_stage(prm, c->p, c->nq, c->cfg, c->stream));
     int rc = read_planes(c, c->Ub, dUdt);
     if (rc) return rc;
     unsigned long long key = 0;
@@ -452,7 +435,7 @@ int vt_stage_kernel_times(vt_ctx* c, double dt, int nsteps, double* ms) {
             prm.mode = vt::kModeRk1 + s;
             prm.stage_id = 4 * i + s;
             HIP_TRY(hipEventRecord(ev[8 * i + 2 * s], c->stream));
-            HIP_TRY(vt::launch_stage(prm, c->p, c->nq, c->stream));
+            HIP_TRY(vt::launch_stage(prm, c->p, c->nq, c->cfg, c->stream));
             HIP_TRY(hipEventRecord(ev[8 * i + 2 * s + 1], c->stream));
         }
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -472,6 +455,25 @@ int vt_stage_kernel_times(vt_ctx* c, double dt, int nsteps, double* ms) {
         decode_error(c, key, 0, 4);
         return VT_ERR_NONPHYSICAL;
     }
+    return VT_OK;
+}
+
+int vt_set_launch(vt_ctx* c, int block, int persistent, int slots_per_xcd) {
+    if (!c || (block != 128 && block != 256) || slots_per_xcd < 0 || (persistent && slots_per_xcd < 1))
+        return VT_ERR_ARG;
+    c->cfg.block = block;
+    c->cfg.persistent = persistent ? 1 : 0;
+    c->cfg.slots_per_xcd = slots_per_xcd;
+    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+    return VT_OK;
+}
+
+int vt_get_launch(const vt_ctx* c, int* cfg) {
+    if (!c || !cfg) return VT_ERR_ARG;
+    cfg[0] = c->cfg.block;
+    cfg[1] = c->cfg.persistent;
+    cfg[2] = c->cfg.slots_per_xcd;
     return VT_OK;
 }
 

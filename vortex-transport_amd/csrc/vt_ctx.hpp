@@ -1,0 +1,63 @@
+// vt_ctx.hpp -- internal: the per-mesh context behind the C ABI and the
+// helpers shared by vt_api.hip and vt_batch.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "vt_api.h"
+#include "vt_kernels.hpp"
+#include "vt_setup.hpp"
+
+struct vt_ctx {
+    int ne = 0, ld = 0, p = 0, np = 0, nq = 0, order = 0, device = 0;
+    hipStream_t stream = nullptr;
+    double *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr, *vtx = nullptr, *tab = nullptr;
+    int* nbr = nullptr;
+    unsigned long long* err = nullptr;       // device error key
+    unsigned long long* err_init = nullptr;  // pinned host, holds kNoError
+    unsigned long long* err_host = nullptr;  // pinned host, one slot per replay
+    size_t err_host_cap = 0;
+    std::vector<double> vertices;
+    double time = 0.0;
+    vt::LaunchConfig cfg;
+    std::map<std::tuple<int, unsigned long long, int>, hipGraphExec_t> graphs;
+    int last_err[6] = {-1, -1, -1, 0, 0, 0};
+};
+
+namespace vti {
+
+// Records a failed HIP call for vt_last_hip_error(); true when e is hipSuccess.
+bool hip_ok(hipError_t e, const char* what);
+
+// Plane stride: element count rounded to 256 plus a 64-element skew, so the
+// 4*Np planes of one element do not all start at the same HBM channel offset.
+inline int plane_stride(int ne) { return ((ne + 255) / 256) * 256 + 64; }
+
+// Graph steps per replay: one graph covers up to kGraphSteps steps.
+constexpr int kGraphSteps = 64;
+
+// Allocates and uploads one mesh (vertices [n][3][2], neighbours [n][3] R,L,V,
+// types [n]) with the given operator table; state zeroed.
+int create_ctx(int n, const double* vertices, const int* neighbors, const int* type, int p, const vt::OperatorTable& tab,
+               vt_ctx** out);
+
+// [n][Np][4] state <-> planes, for elements [e0, e0 + n) of the context.
+void to_planes(const vt_ctx* c, const double* U, int e0, int n, double* planes);
+void from_planes(const vt_ctx* c, const double* planes, int e0, int n, double* U);
+
+// Steps per graph replay for nsteps steps (Euler replays hold an even count,
+// except a single trailing step).
+std::vector<int> replay_plan(int method, int nsteps);
+
+// Pinned host error slots, at least n of them.
+int ensure_err_slots(unsigned long long** slots, size_t* cap, size_t n);
+
+}  // namespace vti
+
+#define HIP_TRY(expr)                                        \
+    do {                                                     \
+        if (!vti::hip_ok((expr), #expr)) return VT_ERR_HIP;  \
+    } while (0)

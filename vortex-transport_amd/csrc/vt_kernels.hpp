@@ -1,13 +1,13 @@
-// vt_kernels.hpp -- launch interface of the stage kernels (vt_kernels.hip).
+// vt_kernels.hpp -- launch interface of the stage kernels (vt_stage.hpp,
+// instantiated by vt_kernels_nq.hip, dispatched by vt_launch.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace vt {
 
-#ifndef VT_BLOCK
-#define VT_BLOCK 256
-#endif
-constexpr int kBlock = VT_BLOCK;
+// Default elements (threads) per workgroup; 128 is the other supported tile.
+constexpr int kBlock = 256;
+constexpr int kMaxGroups = 4;
 
 enum StageMode : int {
     kModeRhs = 0,    // Uout = dU/dt                         (evolve_elem)
@@ -29,12 +29,40 @@ struct StageParams {
     unsigned long long* err;
     double dt;
     int ne, ld, mode, stage_id;
-    int blocks_per_xcd;
+    int blocks_per_xcd;   // tiles per XCD band: ceil(ceil(ne / block) / 8)
+    int err_base;         // added to the element index of an error report
 };
 
-hipError_t launch_stage(const StageParams& prm, int p, int nq, hipStream_t st);
+// Launch shape.  block: elements per workgroup (128 or 256).  persistent:
+// 8 * slots_per_xcd resident workgroups loop over the tiles; otherwise one
+// workgroup per tile.
+struct LaunchConfig {
+    int block = kBlock;
+    int persistent = 0;
+    int slots_per_xcd = 0;
+};
+
+// Several element groups in one launch (instances packed by polynomial
+// order, all with the same quadrature size).  Each group's tiles are split
+// into 8 XCD bands of tpx[g] = ceil(tiles[g] / 8) tiles; XCD x walks the
+// concatenation over groups of its bands, item k of that walk being tile
+// x * tpx[g] + (k - kbegin[g]) of the group g with kbegin[g] <= k <
+// kbegin[g + 1].  Every XCD so gets the same share of each polynomial order.
+// g[i].blocks_per_xcd is unused here.
+struct BatchParams {
+    StageParams g[kMaxGroups];
+    int p[kMaxGroups];
+    int tiles[kMaxGroups];
+    int tpx[kMaxGroups];
+    int kbegin[kMaxGroups + 1];
+    int ngroups;
+};
+
+hipError_t launch_stage(const StageParams& prm, int p, int nq, const LaunchConfig& cfg, hipStream_t st);
+hipError_t launch_batched(const BatchParams& bp, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st);
 
-inline int blocks_per_xcd(int ne) { return ((ne + kBlock - 1) / kBlock + 7) / 8; }
+inline int tiles_per_xcd(int ne, int block) { return ((ne + block - 1) / block + 7) / 8; }
+inline int blocks_per_xcd(int ne) { return tiles_per_xcd(ne, kBlock); }
 
 }  // namespace vt

@@ -1,5 +1,5 @@
-// vt_kernels.hip -- gfx950 kernels for the DG right-hand side and the fused
-// RK4 stages (FP64).
+// vt_stage.hpp -- gfx950 device code for the DG right-hand side and the
+// fused RK4 stages (FP64); instantiated per quadrature size by vt_kernels_nq.hip.
 //
 // One thread per element.  State lives in structure-of-arrays planes
 // U[(k*NP + j)*ld + e] (k = conserved variable, j = node, e = element), so
@@ -118,20 +118,10 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
     return code;
 }
 
-__device__ __forceinline__ void report(unsigned long long* err, int stage, int e, int q, int s, int code) {
+__device__ __forceinline__ void report(unsigned long long* err, int stage, unsigned e, int q, int s, int code) {
     const unsigned long long key = (static_cast<unsigned long long>(stage) << 40) |
                                    ((((static_cast<unsigned long long>(e) * 16 + q) * 4 + s) * 4) + code);
     atomicMin(err, key);
-}
-
-// XCD-aware block order: consecutive workgroups are dispatched round-robin
-// over the 8 XCDs, so block b is remapped to logical block
-// (b % 8) * per_xcd + b / 8.  Each XCD then sweeps one contiguous band of
-// elements, and the vertical neighbours (one element row away) of the band's
-// rows are read through that XCD's own L2.
-__device__ __forceinline__ int logical_block(const StageParams& prm) {
-    const int b = blockIdx.x;
-    return (b & 7) * prm.blocks_per_xcd + (b >> 3);
 }
 
 // Plane access: the plane base is wavefront-uniform (scalar registers) and
@@ -144,19 +134,21 @@ __device__ __forceinline__ double& plane_at(double* base, int plane, size_t ld8,
     return *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + static_cast<size_t>(plane) * ld8 + off8);
 }
 
-template <int P, int NQ>
-__global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams prm) {
+// One tile of BS consecutive elements: BS threads, one element each.  `su`
+// holds at least 4 * NP * BS doubles of LDS.  Every thread of the block
+// calls this (it contains one barrier); threads past the mesh end only take
+// part in the barrier.
+template <int P, int NQ, int BS>
+__device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, double* su) {
     constexpr int NP = (P + 1) * (P + 2) / 2;
     constexpr int NE = P + 1;
     constexpr int NV = 4 * NP;
     constexpr double gmi = 1.4 - 1.0;
     // The block's stage state, one LDS column per thread (value v of thread t
-    // at su[v * kBlock + t]): read once from HBM with all loads in flight
+    // at su[v * BS + t]): read once from HBM with all loads in flight
     // together, then re-read by the volume term, the traces of the element and
     // of its in-block neighbours, and the stage update.
-    __shared__ double su[NV * kBlock];
-    const int lb = logical_block(prm);
-    const int base = lb * kBlock;
+    const int base = tile * BS;
     const int e = base + static_cast<int>(threadIdx.x);
     const bool valid = e < prm.ne;
     const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
@@ -174,7 +166,7 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
     double un[NE][4];
     if (valid) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) my[v * kBlock] = plane_at(Uin, v, ld8, oe);
+        for (int v = 0; v < NV; ++v) my[v * BS] = plane_at(Uin, v, ld8, oe);
         x0 = plane_at(prm.vtx, 0, ld8, oe); y0 = plane_at(prm.vtx, 1, ld8, oe);
         x1 = plane_at(prm.vtx, 2, ld8, oe); y1 = plane_at(prm.vtx, 3, ld8, oe);
         x2 = plane_at(prm.vtx, 4, ld8, oe); y2 = plane_at(prm.vtx, 5, ld8, oe);
@@ -185,7 +177,7 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
             onb[s] = static_cast<unsigned>(nb) * 8u;
         }
         // Edge-node values of the neighbour across side 1, in flight during the volume term.
-        if (static_cast<unsigned>(loc[0]) >= static_cast<unsigned>(kBlock)) {
+        if (static_cast<unsigned>(loc[0]) >= static_cast<unsigned>(BS)) {
 #pragma unroll
             for (int m = 0; m < NE; ++m)
 #pragma unroll
@@ -194,11 +186,11 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
     }
     __syncthreads();
     if (!valid) return;
-    if (static_cast<unsigned>(loc[0]) < static_cast<unsigned>(kBlock)) {
+    if (static_cast<unsigned>(loc[0]) < static_cast<unsigned>(BS)) {
 #pragma unroll
         for (int m = 0; m < NE; ++m)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) un[m][k] = su[(k * NP + edge_node<P>(0, m)) * kBlock + loc[0]];
+            for (int k = 0; k < 4; ++k) un[m][k] = su[(k * NP + edge_node<P>(0, m)) * BS + loc[0]];
     }
 
     // Volume term: r = Sxi G^xi + Seta G^eta with G = adj(J) F (detJ * invJ = adj(J)).
@@ -210,10 +202,10 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
         for (int k = 0; k < 4; ++k) r[i][k] = 0.0;
 #pragma unroll VT_J_UNROLL
     for (int j = 0; j < NP; ++j) {
-        const double rho = my[(0 * NP + j) * kBlock];
-        const double m1 = my[(1 * NP + j) * kBlock];
-        const double m2 = my[(2 * NP + j) * kBlock];
-        const double m3 = my[(3 * NP + j) * kBlock];
+        const double rho = my[(0 * NP + j) * BS];
+        const double m1 = my[(1 * NP + j) * BS];
+        const double m2 = my[(2 * NP + j) * BS];
+        const double m3 = my[(3 * NP + j) * BS];
         const double ir = fast_rcp(rho);
         const double uu = m1 * ir, vv = m2 * ir, E = m3 * ir;
         const double p = rho * gmi * (E - 0.5 * (uu * uu + vv * vv));
@@ -258,14 +250,14 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
                 double a = 0.0, b = 0.0;
 #pragma unroll
                 for (int m = 0; m < NE; ++m) {
-                    a += tp[q * NE + m] * my[(k * NP + edge_node<P>(s, m)) * kBlock];
+                    a += tp[q * NE + m] * my[(k * NP + edge_node<P>(s, m)) * BS];
                     b += tm[q * NE + m] * un[m][k];
                 }
                 UL[k] = a;
                 UR[k] = b;
             }
             const int code = roe_flux(UL, UR, nxv, nyv, F);
-            if (code) report(prm.err, prm.stage_id, e, q, s, code);
+            if (code) report(prm.err, prm.stage_id, prm.err_base + e, q, s, code);
 #pragma unroll
             for (int m = 0; m < NE; ++m) {
                 const double w = len * tl[q * NE + m];
@@ -274,11 +266,11 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
             }
         }
         if (s < 2) {
-            if (static_cast<unsigned>(loc[s + 1]) < static_cast<unsigned>(kBlock)) {
+            if (static_cast<unsigned>(loc[s + 1]) < static_cast<unsigned>(BS)) {
 #pragma unroll
                 for (int m = 0; m < NE; ++m)
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) un[m][k] = su[(k * NP + edge_node<P>(s + 1, m)) * kBlock + loc[s + 1]];
+                    for (int k = 0; k < 4; ++k) un[m][k] = su[(k * NP + edge_node<P>(s + 1, m)) * BS + loc[s + 1]];
             } else {
 #pragma unroll
                 for (int m = 0; m < NE; ++m)
@@ -349,7 +341,7 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
                 const int v = k * NP + i;
                 const double kk = dt * dudt(i, k);
                 plane_at(prm.acc, v, ld8, oe) = kk;
-                plane_at(prm.Uout, v, ld8, oe) = my[v * kBlock] + kk * 0.5;
+                plane_at(prm.Uout, v, ld8, oe) = my[v * BS] + kk * 0.5;
             }
     } else if (mode == kModeEuler) {  // U + dt dU/dt
 #pragma unroll
@@ -357,7 +349,7 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int v = k * NP + i;
-                plane_at(prm.Uout, v, ld8, oe) = my[v * kBlock] + dt * dudt(i, k);
+                plane_at(prm.Uout, v, ld8, oe) = my[v * BS] + dt * dudt(i, k);
             }
     } else {  // kModeRhs
 #pragma unroll
@@ -367,53 +359,100 @@ __global__ void __launch_bounds__(kBlock, VT_MIN_WAVES) stage_kernel(StageParams
     }
 }
 
-// Batch Roe flux, used to check the device flux function on its own.
-__global__ void flux_kernel(const double* __restrict__ in, double* __restrict__ out, int* __restrict__ codes, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double* r = in + 10 * static_cast<size_t>(i);
-    double UL[4] = {r[0], r[1], r[2], r[3]}, UR[4] = {r[4], r[5], r[6], r[7]}, F[4];
-    codes[i] = roe_flux(UL, UR, r[8], r[9], F);
-    for (int k = 0; k < 4; ++k) out[4 * static_cast<size_t>(i) + k] = F[k];
+
+// XCD-aware tile order: consecutive workgroups are dispatched round-robin
+// over the 8 XCDs, so workgroup b works on the band of tiles
+// [(b % 8) * per_xcd, (b % 8 + 1) * per_xcd) of its XCD.  Each XCD then sweeps
+// one contiguous band of elements, and the vertical neighbours (one element
+// row away) of the band's rows are read through that XCD's own L2.
+template <int P, int NQ, int BS>
+__global__ void __launch_bounds__(BS, VT_MIN_WAVES) stage_kernel(StageParams prm) {
+    __shared__ double su[4 * ((P + 1) * (P + 2) / 2) * BS];
+    const int b = blockIdx.x;
+    stage_tile<P, NQ, BS>(prm, (b & 7) * prm.blocks_per_xcd + (b >> 3), su);
 }
 
-template <int P, int NQ>
-static hipError_t launch_pq(const StageParams& prm, int grid, hipStream_t st) {
-    hipLaunchKernelGGL((stage_kernel<P, NQ>), dim3(grid), dim3(kBlock), 0, st, prm);
+// Persistent form: gridDim.x (a multiple of 8) resident workgroups, each
+// looping over the tiles of its XCD's band.
+template <int P, int NQ, int BS>
+__global__ void __launch_bounds__(BS, VT_MIN_WAVES) stage_kernel_persistent(StageParams prm) {
+    __shared__ double su[4 * ((P + 1) * (P + 2) / 2) * BS];
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
+    for (int t = slot; t < prm.blocks_per_xcd; t += slots) {
+        stage_tile<P, NQ, BS>(prm, xcd * prm.blocks_per_xcd + t, su);
+        __syncthreads();
+    }
+}
+
+// Ensemble form: the tiles of up to kMaxGroups element groups (instances
+// packed by polynomial order) in one launch of 8 * slots workgroups; each
+// workgroup takes items slot, slot + slots, ... of its XCD's walk.  With
+// slots = kbegin[ngroups] every workgroup does one tile.  The tile's group
+// selects the P code path (uniform over the workgroup).
+template <int NQ, int BS>
+__global__ void __launch_bounds__(BS, VT_MIN_WAVES) batched_stage_kernel(BatchParams bp) {
+    __shared__ double su[4 * 10 * BS];
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
+    for (int k = slot; k < bp.kbegin[bp.ngroups]; k += slots) {
+        int g = 0;
+#pragma unroll
+        for (int i = 1; i < kMaxGroups; ++i)
+            if (i < bp.ngroups && k >= bp.kbegin[i]) g = i;
+        const int lt = xcd * bp.tpx[g] + (k - bp.kbegin[g]);
+        if (lt < bp.tiles[g]) {
+            switch (bp.p[g]) {
+            case 0: stage_tile<0, NQ, BS>(bp.g[g], lt, su); break;
+            case 1: stage_tile<1, NQ, BS>(bp.g[g], lt, su); break;
+            case 2: stage_tile<2, NQ, BS>(bp.g[g], lt, su); break;
+            default: stage_tile<3, NQ, BS>(bp.g[g], lt, su); break;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Host-side launchers of one quadrature size (defined in vt_kernels_nq.hip).
+template <int NQ>
+struct NqLaunch {
+    static hipError_t stage(const StageParams& prm, int p, const LaunchConfig& cfg, hipStream_t st);
+    static hipError_t batched(const BatchParams& bp, const LaunchConfig& cfg, hipStream_t st);
+};
+
+template <int NQ, int BS, int P>
+hipError_t launch_one(const StageParams& prm, const LaunchConfig& cfg, hipStream_t st) {
+    if (cfg.persistent && cfg.slots_per_xcd > 0 && cfg.slots_per_xcd < prm.blocks_per_xcd) {
+        hipLaunchKernelGGL((stage_kernel_persistent<P, NQ, BS>), dim3(8 * cfg.slots_per_xcd), dim3(BS), 0, st, prm);
+    } else {
+        hipLaunchKernelGGL((stage_kernel<P, NQ, BS>), dim3(8 * prm.blocks_per_xcd), dim3(BS), 0, st, prm);
+    }
     return hipGetLastError();
 }
 
-template <int P>
-static hipError_t launch_p(const StageParams& prm, int nq, int grid, hipStream_t st) {
-    switch (nq) {
-    case 1: return launch_pq<P, 1>(prm, grid, st);
-    case 2: return launch_pq<P, 2>(prm, grid, st);
-    case 3: return launch_pq<P, 3>(prm, grid, st);
-    case 4: return launch_pq<P, 4>(prm, grid, st);
-    case 5: return launch_pq<P, 5>(prm, grid, st);
-    case 6: return launch_pq<P, 6>(prm, grid, st);
-    case 7: return launch_pq<P, 7>(prm, grid, st);
-    case 8: return launch_pq<P, 8>(prm, grid, st);
-    case 9: return launch_pq<P, 9>(prm, grid, st);
-    case 10: return launch_pq<P, 10>(prm, grid, st);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_stage(const StageParams& prm, int p, int nq, hipStream_t st) {
-    const int grid = 8 * prm.blocks_per_xcd;
+template <int NQ, int BS>
+hipError_t launch_bs(const StageParams& prm, int p, const LaunchConfig& cfg, hipStream_t st) {
     switch (p) {
-    case 0: return launch_p<0>(prm, nq, grid, st);
-    case 1: return launch_p<1>(prm, nq, grid, st);
-    case 2: return launch_p<2>(prm, nq, grid, st);
-    case 3: return launch_p<3>(prm, nq, grid, st);
+    case 0: return launch_one<NQ, BS, 0>(prm, cfg, st);
+    case 1: return launch_one<NQ, BS, 1>(prm, cfg, st);
+    case 2: return launch_one<NQ, BS, 2>(prm, cfg, st);
+    case 3: return launch_one<NQ, BS, 3>(prm, cfg, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(flux_kernel, dim3((n + 255) / 256), dim3(256), 0, st, in, out, codes, n);
+template <int NQ>
+hipError_t NqLaunch<NQ>::stage(const StageParams& prm, int p, const LaunchConfig& cfg, hipStream_t st) {
+    return cfg.block == 128 ? launch_bs<NQ, 128>(prm, p, cfg, st) : launch_bs<NQ, 256>(prm, p, cfg, st);
+}
+
+template <int NQ>
+hipError_t NqLaunch<NQ>::batched(const BatchParams& bp, const LaunchConfig& cfg, hipStream_t st) {
+    const int per_xcd = bp.kbegin[bp.ngroups];
+    const int slots = cfg.persistent && cfg.slots_per_xcd > 0 && cfg.slots_per_xcd < per_xcd ? cfg.slots_per_xcd : per_xcd;
+    if (slots < 1) return hipSuccess;
+    if (cfg.block == 128)
+        hipLaunchKernelGGL((batched_stage_kernel<NQ, 128>), dim3(8 * slots), dim3(128), 0, st, bp);
+    else
+        hipLaunchKernelGGL((batched_stage_kernel<NQ, 256>), dim3(8 * slots), dim3(256), 0, st, bp);
     return hipGetLastError();
 }
 
