@@ -141,6 +141,32 @@ int vt_numerical_flux(const double *records, double *flux /* [n][4] */, int *cod
 int vt_set_launch(vt_ctx *ctx, int block, int persistent, int slots_per_xcd);
 int vt_get_launch(const vt_ctx *ctx, int *cfg /* [3] block, persistent, slots_per_xcd */);
 
+/* ---- discrete adjoint ---------------------------------------------------- */
+/* Derivatives of the discrete map of vt_step (method 1): no reference
+ * counterpart (the reference computes none).  f(U) = dU/dt is the right-hand
+ * side of vt_rhs (Evolve.cpp:135-307 assembled by Timestepping.cpp:18-36).  */
+
+/* g = f'(U)^T w at the current state (w, g: [n][Np][4]). */
+int vt_rhs_vjp(vt_ctx *ctx, const double *w, double *g);
+/* Recorded forward run: nsteps RK4 steps of dt from the current state, as
+ * vt_step, keeping the state at the start of every `every` steps on the
+ * device (ceil(nsteps/every) states).  The state ends at U_T. */
+int vt_adjoint_forward(vt_ctx *ctx, double dt, int nsteps, int every);
+/* Reverse sweep over the last recorded forward run: lam0 = (dU_T/dU_0)^T lamT.
+ * Stage states are recomputed one segment (`every` steps) at a time from the
+ * checkpoints; 4*every + 1 states of device memory.  The current state is
+ * not changed. */
+int vt_adjoint_backward(vt_ctx *ctx, const double *lamT, double *lam0);
+/* dU0/d(epsilon, x0, y0) of vt_init_vortex type 0 at time t: dU [3][n][Np][4]. */
+int vt_vortex_tangent(const vt_ctx *ctx, double t, double epsilon, double x0, double y0, double *dU);
+/* J = sum_e detJ_e (rho_e - rho_target_e)^T M_ref (rho_e - rho_target_e) of the
+ * current state, M_ref the reference mass matrix of Preevolve.cpp:43-54;
+ * rho_target [n][Np]; dJdU [n][Np][4] may be NULL. */
+int vt_mass_l2_objective(vt_ctx *ctx, const double *rho_target, double *J, double *dJdU);
+/* vt_adjoint_forward, then J and dJ/dU_T of vt_mass_l2_objective at U_T, then
+ * vt_adjoint_backward: dJdU0 = dJ/dU_0. */
+int vt_adjoint(vt_ctx *ctx, double dt, int nsteps, int every, const double *rho_target, double *J, double *dJdU0);
+
 /* ---- ensemble ------------------------------------------------------------ */
 /* Many independent problems advanced together: no reference counterpart (the
  * reference runs one mesh per process, main.cpp:38-176).  Each instance is

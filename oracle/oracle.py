@@ -60,6 +60,16 @@ def lib():
         L.oro_generate_mesh.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _dp, _ip, _ip]
         L.oro_l2_state_error.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
         L.oro_set_threads.argtypes = [ctypes.c_int]
+        L.oro_roe_flux_jvp.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+        L.oro_roe_flux_vjp.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+        L.oro_rhs_jvp.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.oro_rhs_vjp.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.oro_rk4_tangent.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.oro_rk4_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.oro_vortex_tangent.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, _dp]
+        L.oro_mass_l2_objective.restype = ctypes.c_double
+        L.oro_mass_l2_objective.argtypes = [ctypes.c_void_p, _dp, _dp]
         _lib = L
     return _lib
 
@@ -156,6 +166,22 @@ def numerical_flux(UL, UR, n):
     return out, err
 
 
+def roe_flux_jvp(UL, UR, n, dUL, dUR):
+    """Tangent of the Roe flux (forward mode)."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (UL, UR, n, dUL, dUR)]
+    out = np.zeros(4)
+    lib().oro_roe_flux_jvp(*[_d(x) for x in a], _d(out))
+    return out
+
+
+def roe_flux_vjp(UL, UR, n, bF):
+    """Adjoint of the Roe flux (reverse mode): (bUL, bUR)."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (UL, UR, n, bF)]
+    bl, br = np.zeros(4), np.zeros(4)
+    lib().oro_roe_flux_vjp(*[_d(x) for x in a], _d(bl), _d(br))
+    return bl, br
+
+
 def l2_state_error(vertices, U_first, U_last):
     v = np.ascontiguousarray(vertices, dtype=np.float64)
     a = np.ascontiguousarray(U_first, dtype=np.float64)
@@ -226,6 +252,41 @@ class Solver:
         d = np.zeros((self.ne, self.np, 4))
         rc = lib().oro_evolve_elem(self.h, _d(d))
         return d, rc
+
+    def rhs_jvp(self, dU):
+        dU = np.ascontiguousarray(dU, dtype=np.float64)
+        out = np.zeros_like(dU)
+        lib().oro_rhs_jvp(self.h, _d(dU), _d(out))
+        return out
+
+    def rhs_vjp(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        out = np.zeros_like(w)
+        lib().oro_rhs_vjp(self.h, _d(w), _d(out))
+        return out
+
+    def rk4_tangent(self, dt, nsteps, dU0):
+        dU0 = np.ascontiguousarray(dU0, dtype=np.float64)
+        out = np.zeros_like(dU0)
+        lib().oro_rk4_tangent(self.h, dt, nsteps, _d(dU0), _d(out))
+        return out
+
+    def rk4_adjoint(self, dt, nsteps, lamT):
+        lamT = np.ascontiguousarray(lamT, dtype=np.float64)
+        out = np.zeros_like(lamT)
+        assert lib().oro_rk4_adjoint(self.h, dt, nsteps, _d(lamT), _d(out)) == 0
+        return out
+
+    def vortex_tangent(self, t=0.0, epsilon=0.3, x0=0.0, y0=0.0):
+        out = np.zeros((3, self.ne, self.np, 4))
+        lib().oro_vortex_tangent(self.h, t, epsilon, x0, y0, _d(out))
+        return out
+
+    def mass_l2_objective(self, rho_target):
+        rt = np.ascontiguousarray(rho_target, dtype=np.float64).reshape(self.ne, self.np)
+        g = np.zeros((self.ne, self.np, 4))
+        J = lib().oro_mass_l2_objective(self.h, _d(rt), _d(g))
+        return J, g
 
     def run(self, method, dt, nsteps):
         return lib().oro_run(self.h, method, dt, nsteps)

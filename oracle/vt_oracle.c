@@ -10,6 +10,7 @@
  */
 #define _DEFAULT_SOURCE
 #include "vt_oracle.h"
+#include "oro_internal.h"
 
 #include <math.h>
 #include <stdint.h>
@@ -21,8 +22,6 @@
 
 #include "oro_tables.h"
 
-#define MAXNP 10
-#define MAXNQ 10
 
 static int g_threads = 0;
 void oro_set_threads(int n) { g_threads = n; }
@@ -299,32 +298,6 @@ int oro_generate_mesh(int nx, int ny, float domain_x, float domain_y, float pert
     free(gx); free(gy); free(px); free(py); free(n1); free(n2); free(n3);
     return rc;
 }
-
-struct oro_solver {
-    int ne, p, np, nq;
-    double xw[MAXNQ][2];
-    double *vert;        /* [ne][6] */
-    int *bnd, *type;     /* [ne][3], [ne] */
-    double *nodes_phys;  /* [ne][np][2] */
-    double *geo;         /* [ne][16]: jac4 detJ invJ4 detInvJ n6 ... */
-    double *len;         /* [ne][3] */
-    double Minv_ref[MAXNP * MAXNP];
-    double S_ref[2 * MAXNP * MAXNP];
-    double *Minv_phys;   /* [ne][np][np] */
-    double *S_phys;      /* [ne][2][np][np] */
-    double plus[3][MAXNP][MAXNQ], minus[3][MAXNP][MAXNQ];
-    double *U, *F, *time, *dU;
-    int order;
-    int err[6];        /* elem, q, side, code, step, stage of the first failure */
-    int cur_step, cur_stage;
-};
-
-#define GEO_JAC 0
-#define GEO_DET 4
-#define GEO_INVJ 5
-#define GEO_DETINV 9
-#define GEO_N 10
-#define GEO_STRIDE 16
 
 int oro_num_elements(const oro_solver *s) { return s->ne; }
 int oro_num_nodes(const oro_solver *s) { return s->np; }

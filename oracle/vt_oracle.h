@@ -105,6 +105,25 @@ void oro_l2_state_error(int ne, const double *vertices, const double *U_first,
 /* Number of OpenMP threads the solver uses (<=0: runtime default). */
 void oro_set_threads(int n);
 
+/* ---- derivatives (vt_oracle_adjoint.c) ----------------------------------
+ * Tangents are forward mode, adjoints reverse mode, written separately.  The
+ * right-hand side f(U) = dU/dt is taken at the solver's current state. */
+void oro_roe_flux_jvp(const double *UL, const double *UR, const double *n, const double *dUL,
+                      const double *dUR, double *dF);
+void oro_roe_flux_vjp(const double *UL, const double *UR, const double *n, const double *bF,
+                      double *bUL, double *bUR);
+void oro_rhs_jvp(const oro_solver *s, const double *dU, double *df);  /* df = f'(U) dU */
+void oro_rhs_vjp(const oro_solver *s, const double *w, double *g);    /* g = f'(U)^T w */
+/* nsteps RK4 steps of dt from the current state (which ends at U_T):
+ * dUT = (dU_T/dU_0) dU0, lam0 = (dU_T/dU_0)^T lamT. */
+int oro_rk4_tangent(oro_solver *s, double dt, int nsteps, const double *dU0, double *dUT);
+int oro_rk4_adjoint(oro_solver *s, double dt, int nsteps, const double *lamT, double *lam0);
+/* d U0 / d(epsilon, x0, y0) of the type 0 vortex state at time t: [3][ne][Np][4]. */
+void oro_vortex_tangent(const oro_solver *s, double t, double epsilon, double x0, double y0, double *dU);
+/* J = sum_e detJ_e drho_e^T M_ref drho_e with drho = rho(current) - rho_target
+ * ([ne][Np]); dJdU ([ne][Np][4], may be NULL) receives its gradient. */
+double oro_mass_l2_objective(const oro_solver *s, const double *rho_target, double *dJdU);
+
 #ifdef __cplusplus
 }
 #endif

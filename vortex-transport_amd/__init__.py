@@ -96,6 +96,12 @@ def lib():
             "vt_step_timed": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
             "vt_stage_kernel_times": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, _dp]),
             "vt_numerical_flux": (ctypes.c_int, [_dp, _dp, _ip, ctypes.c_int]),
+            "vt_rhs_vjp": (ctypes.c_int, [vp, _dp, _dp]),
+            "vt_adjoint_forward": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+            "vt_adjoint_backward": (ctypes.c_int, [vp, _dp, _dp]),
+            "vt_vortex_tangent": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
+            "vt_mass_l2_objective": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+            "vt_adjoint": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
             "vt_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "vt_get_launch": (ctypes.c_int, [vp, _ip]),
             "vt_batch_create": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip, _dp, _ip, _ip, ctypes.POINTER(vp)]),
@@ -384,6 +390,48 @@ class Solver:
 
     def set_launch(self, block=256, persistent=False, slots_per_xcd=0):
         _check(lib().vt_set_launch(self._h, block, int(bool(persistent)), slots_per_xcd))
+
+    # --- discrete adjoint (no reference counterpart) ---------------------
+
+    def rhs_vjp(self, w):
+        """g = f'(U)^T w at the current state, f = dU/dt of evolve_elem."""
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.ne, self.np, 4)
+        g = np.zeros_like(w)
+        _check(lib().vt_rhs_vjp(self._h, _d(w), _d(g)))
+        return g
+
+    def adjoint_forward(self, dt, nsteps, every=32):
+        """RK4 steps as rk4(), recording a checkpoint every `every` steps."""
+        _check(lib().vt_adjoint_forward(self._h, dt, nsteps, every), self)
+
+    def adjoint_backward(self, lamT):
+        """(dU_T/dU_0)^T lamT over the last recorded forward run."""
+        lamT = np.ascontiguousarray(lamT, dtype=np.float64).reshape(self.ne, self.np, 4)
+        lam0 = np.zeros_like(lamT)
+        _check(lib().vt_adjoint_backward(self._h, _d(lamT), _d(lam0)))
+        return lam0
+
+    def vortex_tangent(self, t=0.0, epsilon=0.3, x0=0.0, y0=0.0):
+        """dU0/d(epsilon, x0, y0) of the type 0 vortex state: [3][n][Np][4]."""
+        out = np.zeros((3, self.ne, self.np, 4))
+        _check(lib().vt_vortex_tangent(self._h, t, epsilon, x0, y0, _d(out)))
+        return out
+
+    def mass_l2_objective(self, rho_target):
+        """(J, dJ/dU) of J = sum_e detJ_e drho^T M_ref drho at the current state."""
+        rt = np.ascontiguousarray(rho_target, dtype=np.float64).reshape(self.ne, self.np)
+        J = ctypes.c_double()
+        g = np.zeros((self.ne, self.np, 4))
+        _check(lib().vt_mass_l2_objective(self._h, _d(rt), ctypes.byref(J), _d(g)))
+        return J.value, g
+
+    def adjoint(self, dt, nsteps, rho_target, every=32):
+        """Forward run, objective and reverse sweep: (J, dJ/dU0)."""
+        rt = np.ascontiguousarray(rho_target, dtype=np.float64).reshape(self.ne, self.np)
+        J = ctypes.c_double()
+        g = np.zeros((self.ne, self.np, 4))
+        _check(lib().vt_adjoint(self._h, dt, nsteps, every, _d(rt), ctypes.byref(J), _d(g)), self)
+        return J.value, g
 
     def launch(self):
         c = np.zeros(3, dtype=np.int32)

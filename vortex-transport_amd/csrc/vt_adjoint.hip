@@ -1,0 +1,229 @@
+// vt_adjoint.hip -- C ABI of the discrete adjoint of the RK4 loop: recorded
+// forward runs with checkpoints, the checkpointed reverse sweep, the adjoint
+// of a single right-hand side, and host helpers (vortex initial-state tangent,
+// mass-weighted L2 objective).  No reference counterpart: the reference has no
+// derivatives.
+//
+// Memory (one "slot" = one state, 4 * Np * ld doubles):
+//   ckpt      ceil(nsteps / every) slots, the state at the start of each segment;
+//   seg       4 * every + 1 slots, the four stage states of every step of one
+//             segment (recomputed from its checkpoint during the reverse sweep);
+//   adj       4 slots: lam, acc, y (ping-pong pair).
+// The reverse sweep launches, per segment, the forward stage kernels of the
+// segment with their outputs directed into seg (no extra copies), then per step
+// the four adjoint stage kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "vt_ctx.hpp"
+#include "vt_device.hpp"
+
+namespace {
+
+using vti::hip_ok;
+
+size_t slot_elems(const vt_ctx* c) { return static_cast<size_t>(4) * c->np * c->ld; }
+
+int ensure_adj_planes(vt_ctx* c) {
+    if (c->adj) return VT_OK;
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->adj), sizeof(double) * 4 * slot_elems(c)));
+    HIP_TRY(hipMemset(c->adj, 0, sizeof(double) * 4 * slot_elems(c)));
+    return VT_OK;
+}
+
+vt::AdjParams adj_params(vt_ctx* c, int mode) {
+    vt::AdjParams a{};
+    a.vtx = c->vtx;
+    a.nbr = c->nbr;
+    a.tab = c->tab;
+    a.ne = c->ne;
+    a.ld = c->ld;
+    a.mode = mode;
+    a.blocks_per_xcd = ((c->ne + vt::kAdjBlock - 1) / vt::kAdjBlock + 7) / 8;
+    return a;
+}
+
+int upload_planes(vt_ctx* c, const double* U, double* dev) {
+    std::vector<double> h(slot_elems(c), 0.0);
+    vti::to_planes(c, U, 0, c->ne, h.data());
+    HIP_TRY(hipMemcpyAsync(dev, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
+int download_planes(vt_ctx* c, const double* dev, double* U) {
+    std::vector<double> h(slot_elems(c));
+    HIP_TRY(hipMemcpyAsync(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    vti::from_planes(c, h.data(), 0, c->ne, U);
+    return VT_OK;
+}
+
+// Forward stage kernels of one step from slot S0 into slots S1, S2, S3 and next.
+hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* S2, double* S3, double* next) {
+    vt::StageParams prm{};
+    prm.vtx = c->vtx;
+    prm.nbr = c->nbr;
+    prm.tab = c->tab;
+    prm.err = c->err;
+    prm.ne = c->ne;
+    prm.ld = c->ld;
+    prm.blocks_per_xcd = vt::tiles_per_xcd(c->ne, c->cfg.block);
+    prm.dt = dt;
+    prm.acc = c->acc;
+    const double* in[4] = {S0, S1, S2, S3};
+    double* out[4] = {S1, S2, S3, next};
+    for (int s = 0; s < 4; ++s) {
+        prm.Uin = in[s];
+        prm.Uout = out[s];
+        prm.Un = (s == 1 || s == 2) ? S0 : nullptr;
+        prm.mode = vt::kModeRk1 + s;
+        prm.stage_id = s;
+        hipError_t e = vt::launch_stage(prm, c->p, c->nq, c->cfg, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+void vti::free_adjoint(vt_ctx* c) {
+    for (double* p : c->ckpt)
+        if (p) (void)hipFree(p);
+    c->ckpt.clear();
+    if (c->seg) (void)hipFree(c->seg);
+    if (c->adj) (void)hipFree(c->adj);
+    c->seg = nullptr;
+    c->adj = nullptr;
+    c->seg_slots = 0;
+    c->recorded = false;
+}
+
+extern "C" {
+
+int vt_rhs_vjp(vt_ctx* c, const double* w, double* g) {
+    if (!c || !w || !g) return VT_ERR_ARG;
+    int rc = ensure_adj_planes(c);
+    if (rc) return rc;
+    const size_t n = slot_elems(c);
+    double *lam = c->adj, *ya = c->adj + 2 * n, *yb = c->adj + 3 * n;
+    rc = upload_planes(c, w, lam);
+    if (rc) return rc;
+    vt::AdjParams a = adj_params(c, vt::kAdjSeed);
+    a.lam = lam;
+    a.Yout = ya;
+    a.scale = 1.0;
+    HIP_TRY(vt::launch_adjoint(a, c->p, c->nq, c->stream));
+    a = adj_params(c, vt::kAdjRaw);
+    a.U = c->Un;
+    a.Y = ya;
+    a.Yout = yb;
+    HIP_TRY(vt::launch_adjoint(a, c->p, c->nq, c->stream));
+    return download_planes(c, yb, g);
+}
+
+int vt_adjoint_forward(vt_ctx* c, double dt, int nsteps, int every) {
+    if (!c || nsteps < 1 || every < 1) return VT_ERR_ARG;
+    c->recorded = false;
+    const int nseg = (nsteps + every - 1) / every;
+    const size_t n = slot_elems(c);
+    if (static_cast<int>(c->ckpt.size()) < nseg) {
+        for (int i = static_cast<int>(c->ckpt.size()); i < nseg; ++i) {
+            double* p = nullptr;
+            HIP_TRY(hipMalloc(reinterpret_cast<void**>(&p), sizeof(double) * n));
+            c->ckpt.push_back(p);
+        }
+    }
+    for (int i = 0; i < nseg; ++i) {
+        const int len = (i + 1 < nseg) ? every : nsteps - i * every;
+        HIP_TRY(hipMemcpyAsync(c->ckpt[i], c->Un, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        int rc = vti::run_steps(c, 1, dt, len, nullptr, nullptr);
+        if (rc) {
+            if (rc == VT_ERR_NONPHYSICAL) c->last_err[4] += i * every;
+            return rc;
+        }
+    }
+    c->rec_steps = nsteps;
+    c->rec_every = every;
+    c->rec_dt = dt;
+    c->recorded = true;
+    return VT_OK;
+}
+
+int vt_adjoint_backward(vt_ctx* c, const double* lamT, double* lam0) {
+    if (!c || !lamT || !lam0) return VT_ERR_ARG;
+    if (!c->recorded) return VT_ERR_ARG;
+    int rc = ensure_adj_planes(c);
+    if (rc) return rc;
+    const int every = c->rec_every, nsteps = c->rec_steps, nseg = (nsteps + every - 1) / every;
+    const double dt = c->rec_dt;
+    const size_t n = slot_elems(c);
+    const int need = 4 * (nsteps < every ? nsteps : every) + 1;
+    if (c->seg_slots < need) {
+        if (c->seg) (void)hipFree(c->seg);
+        c->seg = nullptr;
+        c->seg_slots = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->seg), sizeof(double) * n * need));
+        c->seg_slots = need;
+    }
+    double *lam = c->adj, *acc = c->adj + n, *y[2] = {c->adj + 2 * n, c->adj + 3 * n};
+    rc = upload_planes(c, lamT, lam);
+    if (rc) return rc;
+    vt::AdjParams a = adj_params(c, vt::kAdjSeed);
+    a.lam = lam;
+    a.Yout = y[0];
+    a.scale = 1.0 / 6.0;
+    HIP_TRY(vt::launch_adjoint(a, c->p, c->nq, c->stream));
+    int cur = 0;
+    auto slot = [&](int step, int stage) { return c->seg + (static_cast<size_t>(4) * step + stage) * n; };
+    for (int i = nseg - 1; i >= 0; --i) {
+        const int len = (i + 1 < nseg) ? every : nsteps - i * every;
+        HIP_TRY(hipMemcpyAsync(slot(0, 0), c->ckpt[i], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        for (int s = 0; s < len; ++s)
+            HIP_TRY(recompute_step(c, dt, slot(s, 0), slot(s, 1), slot(s, 2), slot(s, 3), slot(s + 1, 0)));
+        for (int s = len - 1; s >= 0; --s)
+            for (int st = 3; st >= 0; --st) {
+                vt::AdjParams b = adj_params(c, vt::kAdj1 + st);
+                b.U = slot(s, st);
+                b.Y = y[cur];
+                b.Yout = y[cur ^ 1];
+                b.lam = lam;
+                b.acc = acc;
+                b.dt = dt;
+                HIP_TRY(vt::launch_adjoint(b, c->p, c->nq, c->stream));
+                cur ^= 1;
+            }
+    }
+    return download_planes(c, lam, lam0);
+}
+
+int vt_vortex_tangent(const vt_ctx* c, double t, double epsilon, double x0, double y0, double* dU) {
+    if (!c || !dU) return VT_ERR_ARG;
+    vt::vortex_tangent(c->ne, c->vertices.data(), c->p, t, epsilon, x0, y0, dU);
+    return VT_OK;
+}
+
+int vt_mass_l2_objective(vt_ctx* c, const double* rho_target, double* J, double* dJdU) {
+    if (!c || !rho_target || !J) return VT_ERR_ARG;
+    if (c->order < 0) return VT_ERR_QUAD;
+    std::vector<double> M(static_cast<size_t>(c->np) * c->np), U(static_cast<size_t>(c->ne) * c->np * 4);
+    if (!vt::reference_operators(c->p, c->order, M.data(), nullptr, nullptr)) return VT_ERR_QUAD;
+    int rc = vt_get_state(c, U.data());
+    if (rc) return rc;
+    *J = vt::mass_l2_objective(c->ne, c->vertices.data(), c->p, M.data(), U.data(), rho_target, dJdU);
+    return VT_OK;
+}
+
+int vt_adjoint(vt_ctx* c, double dt, int nsteps, int every, const double* rho_target, double* J, double* dJdU0) {
+    if (!c || !rho_target || !J || !dJdU0) return VT_ERR_ARG;
+    int rc = vt_adjoint_forward(c, dt, nsteps, every);
+    if (rc) return rc;
+    std::vector<double> g(static_cast<size_t>(c->ne) * c->np * 4);
+    rc = vt_mass_l2_objective(c, rho_target, J, g.data());
+    if (rc) return rc;
+    return vt_adjoint_backward(c, g.data(), dJdU0);
+}
+
+}  // extern "C"

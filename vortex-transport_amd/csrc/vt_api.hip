@@ -152,7 +152,9 @@ void decode_error(vt_ctx* c, unsigned long long key, int step_base, int per_step
     c->last_err[5] = static_cast<int>(stage % per_step) + 1;
 }
 
-int run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0, hipEvent_t ev1) {
+}  // namespace
+
+int vti::run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0, hipEvent_t ev1) {
     if (method != 0 && method != 1) return VT_ERR_METHOD;
     if (nsteps < 0) return VT_ERR_ARG;
     const std::vector<int> reps = vti::replay_plan(method, nsteps);
@@ -189,8 +191,6 @@ int run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0, hipE
     }
     return VT_OK;
 }
-
-}  // namespace
 
 extern "C" {
 
@@ -319,6 +319,7 @@ int vt_create_with_operators(int n, const double* vertices, const int* neighbors
 
 void vt_destroy(vt_ctx* c) {
     if (!c) return;
+    vti::free_adjoint(c);
     for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
     for (void* b : {static_cast<void*>(c->Un), static_cast<void*>(c->Ua), static_cast<void*>(c->Ub),
                     static_cast<void*>(c->acc), static_cast<void*>(c->vtx), static_cast<void*>(c->nbr),
@@ -400,7 +401,7 @@ int vt_rhs(vt_ctx* c, double* dUdt) {
 
 int vt_step(vt_ctx* c, int method, double dt, int nsteps) {
     if (!c) return VT_ERR_ARG;
-    return run_steps(c, method, dt, nsteps, nullptr, nullptr);
+    return vti::run_steps(c, method, dt, nsteps, nullptr, nullptr);
 }
 
 int vt_step_timed(vt_ctx* c, int method, double dt, int nsteps, double* ms) {
@@ -408,7 +409,7 @@ int vt_step_timed(vt_ctx* c, int method, double dt, int nsteps, double* ms) {
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    int rc = run_steps(c, method, dt, nsteps, e0, e1);
+    int rc = vti::run_steps(c, method, dt, nsteps, e0, e1);
     float f = 0.f;
     if (rc == VT_OK && !hip_ok(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime")) rc = VT_ERR_HIP;
     *ms = f;

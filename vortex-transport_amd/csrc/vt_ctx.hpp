@@ -25,6 +25,15 @@ struct vt_ctx {
     vt::LaunchConfig cfg;
     std::map<std::tuple<int, unsigned long long, int>, hipGraphExec_t> graphs;
     int last_err[6] = {-1, -1, -1, 0, 0, 0};
+    // Discrete adjoint (vt_adjoint.hip): checkpoints of the recorded forward
+    // run, the stage states of one segment, and lam / acc / y ping-pong planes.
+    std::vector<double*> ckpt;
+    double* seg = nullptr;
+    double* adj = nullptr;
+    int seg_slots = 0;
+    int rec_steps = 0, rec_every = 0;
+    double rec_dt = 0.0;
+    bool recorded = false;
 };
 
 namespace vti {
@@ -51,6 +60,13 @@ void from_planes(const vt_ctx* c, const double* planes, int e0, int n, double* U
 // Steps per graph replay for nsteps steps (Euler replays hold an even count,
 // except a single trailing step).
 std::vector<int> replay_plan(int method, int nsteps);
+
+// nsteps RK4 (method 1) or Euler (0) steps through graph replays, with
+// optional events around them; VT_ERR_NONPHYSICAL locates the first failure.
+int run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0, hipEvent_t ev1);
+
+// Frees the adjoint recording of a context.
+void free_adjoint(vt_ctx* c);
 
 // Pinned host error slots, at least n of them.
 int ensure_err_slots(unsigned long long** slots, size_t* cap, size_t n);

@@ -58,7 +58,27 @@ struct BatchParams {
     int ngroups;
 };
 
+// Discrete adjoint (vt_adjoint.hpp).  Modes: kAdjRaw writes g = f'(U)^T w to
+// Yout; kAdj4..kAdj1 are the four stages of one reverse RK4 step; kAdjSeed
+// writes Yout = scale * Minv_ref^T lam / detJ (no flux work).
+enum AdjMode : int { kAdjRaw = 0, kAdj1 = 1, kAdj2 = 2, kAdj3 = 3, kAdj4 = 4, kAdjSeed = 5 };
+constexpr int kAdjBlock = 256;
+
+struct AdjParams {
+    const double* U;      // stage state the right-hand side is linearised at
+    const double* Y;      // y = Minv_ref^T w / detJ of the stage's cotangent w
+    double* Yout;         // y of the next stage's cotangent (kAdjRaw: g)
+    double* lam;          // dJ/dU_{n+1}; kAdj1 overwrites it with dJ/dU_n; kAdjSeed input
+    double* acc;          // sum of the stage contributions
+    const double* vtx;
+    const int* nbr;
+    const double* tab;
+    double dt, scale;
+    int ne, ld, mode, blocks_per_xcd;
+};
+
 hipError_t launch_stage(const StageParams& prm, int p, int nq, const LaunchConfig& cfg, hipStream_t st);
+hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st);
 hipError_t launch_batched(const BatchParams& bp, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st);
 

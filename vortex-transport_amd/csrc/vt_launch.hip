@@ -37,6 +37,15 @@ hipError_t launch_batched(const BatchParams& bp, int nq, const LaunchConfig& cfg
     }
 }
 
+hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st) {
+    switch (nq) {
+#define X(n) case n: return NqLaunch<n>::adjoint(prm, p, st);
+        VT_NQ_CASES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(flux_kernel, dim3((n + 255) / 256), dim3(256), 0, st, in, out, codes, n);

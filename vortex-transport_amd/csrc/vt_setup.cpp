@@ -409,4 +409,67 @@ bool build_operator_table(int p, int order, OperatorTable* out) {
     return true;
 }
 
+
+void vortex_tangent(int ne, const double* V, int p, double t, double epsilon, double x0, double y0, double* dU) {
+    const int np = num_nodes(p);
+    const double gamma = 1.4, M_infty = 0.5, p_infty = 20.0 / 7.0, rc = 1.0;
+    const double U_infty = 1.0 / std::sqrt(2.0), V_infty = 1.0 / std::sqrt(2.0);
+    const double A = (gamma - 1) * M_infty * M_infty / (8.0 * M_PI * M_PI);
+    const double B = (U_infty * U_infty + V_infty * V_infty) / (2.0 * M_PI * rc);
+    const size_t plane = static_cast<size_t>(ne) * np * 4;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) {
+        double xy[2 * kMaxNp];
+        element_nodes(p, V + 6 * e, xy);
+        for (int i = 0; i < np; ++i) {
+            const double dx = xy[2 * i] - x0 - U_infty * t, dy = xy[2 * i + 1] - y0 - V_infty * t;
+            const double f0 = 1.0 - (dx * dx + dy * dy) / (rc * rc);
+            const double ex = std::exp(f0), eh = std::exp(0.5 * f0);
+            const double f1 = 1.0 - epsilon * epsilon * A * ex;
+            const double f2 = epsilon * B * eh;
+            const double rho = std::pow(f1, 1.0 / (gamma - 1.0)), pr = p_infty * std::pow(f1, gamma / (gamma - 1.0));
+            const double u = U_infty - f2 * dy, v = V_infty + f2 * dx;
+            // d f0 / d(eps, x0, y0) = (0, 2 dx, 2 dy); d dx/dx0 = -1, d dy/dy0 = -1
+            const double df0[3] = {0.0, 2.0 * dx / (rc * rc), 2.0 * dy / (rc * rc)};
+            for (int a = 0; a < 3; ++a) {
+                const double df1 = -(a == 0 ? 2.0 * epsilon : 0.0) * A * ex - epsilon * epsilon * A * ex * df0[a];
+                const double df2 = (a == 0 ? B * eh : 0.0) + f2 * 0.5 * df0[a];
+                const double drho = rho / ((gamma - 1.0) * f1) * df1;
+                const double dp = pr * gamma / ((gamma - 1.0) * f1) * df1;
+                const double ddx = a == 1 ? -1.0 : 0.0, ddy = a == 2 ? -1.0 : 0.0;
+                const double du = -df2 * dy - f2 * ddy, dv = df2 * dx + f2 * ddx;
+                double* o = dU + a * plane + (static_cast<size_t>(e) * np + i) * 4;
+                o[0] = drho;
+                o[1] = drho * u + rho * du;
+                o[2] = drho * v + rho * dv;
+                o[3] = dp / (gamma - 1.0) + 0.5 * drho * (u * u + v * v) + rho * (u * du + v * dv);
+            }
+        }
+    }
+}
+
+double mass_l2_objective(int ne, const double* V, int p, const double* M, const double* U, const double* rt,
+                         double* dJdU) {
+    const int np = num_nodes(p);
+    double J = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : J)
+    for (int e = 0; e < ne; ++e) {
+        const double* v = V + 6 * e;
+        const double det = (v[2] - v[0]) * (v[5] - v[1]) - (v[3] - v[1]) * (v[4] - v[0]);
+        double d[kMaxNp];
+        for (int i = 0; i < np; ++i) d[i] = U[(static_cast<size_t>(e) * np + i) * 4] - rt[static_cast<size_t>(e) * np + i];
+        for (int i = 0; i < np; ++i) {
+            double a = 0.0;
+            for (int j = 0; j < np; ++j) a += M[i * np + j] * d[j];
+            J += det * d[i] * a;
+            if (dJdU) {
+                double* g = dJdU + (static_cast<size_t>(e) * np + i) * 4;
+                g[0] = 2.0 * det * a;
+                g[1] = g[2] = g[3] = 0.0;
+            }
+        }
+    }
+    return J;
+}
+
 }  // namespace vt

@@ -57,6 +57,14 @@ void element_nodes(int p, const double* v /*[3][2]*/, double* xy /*[Np][2]*/);
 // U in [ne][Np][4] order.  Returns false for an unknown init type.
 bool initial_state(int ne, const double* vertices, int p, int order, int init_type, double epsilon,
                    double x0, double y0, double t, double* U);
+// Derivative of the type 0 nodal vortex state at time t with respect to
+// (epsilon, x0, y0): dU [3][ne][Np][4] (closed form of Element.cpp:134-166).
+void vortex_tangent(int ne, const double* vertices, int p, double t, double epsilon, double x0, double y0,
+                    double* dU);
+// J = sum_e detJ_e drho_e^T M_ref drho_e, drho = U[:, :, 0] - rho_target ([ne][Np]);
+// dJdU [ne][Np][4] (may be null) receives the gradient.  M_ref: [Np][Np].
+double mass_l2_objective(int ne, const double* vertices, int p, const double* M_ref, const double* U,
+                         const double* rho_target, double* dJdU);
 // Nodal flux from a state, the refresh of Timestepping.cpp:59-76.
 void nodal_flux(const double* U /*[4]*/, double* F /*[2][4]*/);
 

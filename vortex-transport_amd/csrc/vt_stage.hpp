@@ -18,6 +18,7 @@
 //   * performs the RK4 stage update (Timestepping.cpp:85-305) in registers.
 // Reference operators and edge tables are uniform across the grid and read
 // with scalar loads from one packed table.
+#pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -416,6 +417,7 @@ template <int NQ>
 struct NqLaunch {
     static hipError_t stage(const StageParams& prm, int p, const LaunchConfig& cfg, hipStream_t st);
     static hipError_t batched(const BatchParams& bp, const LaunchConfig& cfg, hipStream_t st);
+    static hipError_t adjoint(const AdjParams& prm, int p, hipStream_t st);
 };
 
 template <int NQ, int BS, int P>
