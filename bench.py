@@ -11,6 +11,11 @@ numerics leave the physical range in step 4 (its Roe eigenvalues use
 ci = c2^2 ~ 16), after which the reference exits.  The work per step does not
 depend on dt.
 
+Beside the plain figure the line carries "batched" (the 64-instance
+ensemble, one launch per RK stage, against per-instance plain solvers) and
+"adjoint" (forward + checkpointed discrete adjoint on 256 x 256 cells, against
+the forward solve alone).
+
 Prints one JSON line (rank 0).  --gpus N > 1 exits: this build runs on one GPU.
 """
 import argparse
@@ -46,6 +51,10 @@ def parse_args():
     ap.add_argument("--kernel-steps", type=int, default=10, help="steps timed per stage launch for the roofline")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=64)
+    ap.add_argument("--adjoint-steps", type=int, default=1000)
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 --pmc: 2 steps of per-launch stage kernels, then a plain device copy "
                          "of 2^27 doubles (1 GiB read + 1 GiB written) to calibrate FETCH_SIZE / WRITE_SIZE")
@@ -77,6 +86,78 @@ def traffic_from_profile(n_elem):
     if not m or not e or int(e.group(1)) != n_elem:
         return None
     return float(m.group(1))
+
+
+def batched_figure(vt, steps, dt=1e-4):
+    """Ensemble of 64 instances (seed 3: cells per side 32..256, p 1..3,
+    epsilon 0.1..0.4, centre in [-2.5, 2.5]^2 of a 10 x 10 domain), one batched
+    launch per RK stage with the autotuned launch shape, against the same 64
+    instances stepped one plain solver after another."""
+    import torch
+    draws = vt.ensemble_draws(3, 64)
+    meshes = [vt.generate_mesh(nx=d["cells"], ny=d["cells"]) for d in draws]
+    ps = [d["p"] for d in draws]
+    args = ([d["epsilon"] for d in draws], [d["x0"] for d in draws], [d["y0"] for d in draws])
+    b = vt.Batch(meshes, ps, 8)
+    b.initialize_hydrodinamics(0, *args)
+    cfg, _ = b.autotune(dt, 8)
+    b.rk4(dt, 4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.rk4(dt, steps)
+    wb = time.perf_counter() - t0
+    solvers = []
+    for m, d in zip(meshes, draws):
+        s = vt.Solver(m, d["p"], 8)
+        s.initialize_hydrodinamics(0, d["epsilon"], d["x0"], d["y0"])
+        s.rk4(dt, 4)
+        solvers.append(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.rk4(dt, steps)
+    wp = time.perf_counter() - t0
+    worst = max(float(np.linalg.norm(b.state(i) - s.state()) / np.linalg.norm(s.state())) for i, s in enumerate(solvers))
+    dof = sum(2 * d["cells"] ** 2 * (d["p"] + 1) * (d["p"] + 2) // 2 * 16 for d in draws)
+    for s in solvers:
+        s.close()
+    launches = b.num_launches
+    b.close()
+    return {"workload": "64 instances (seed 3): cells 32..256 per side, p 1..3, eps 0.1..0.4, 10x10 domain, "
+                        "integration_order 8, RK4, dt %g" % dt,
+            "elements": sum(2 * d["cells"] ** 2 for d in draws), "steps": steps,
+            "ms_per_step": wb / steps * 1e3, "value": dof * steps / wb, "unit": "DOF-stage updates/s",
+            "per_instance_ms_per_step": wp / steps * 1e3, "speedup_vs_per_instance": wp / wb,
+            "launch": cfg, "launches_per_stage": launches, "max_rel_l2_vs_per_instance": worst}
+
+
+def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
+    """Forward + discrete adjoint (checkpoints every 32 steps) of the
+    mass-weighted L2 misfit to the advected target vortex (eps 0.32, centre
+    (0.1, -0.05)) on n x n cells of a 10 x 10 domain, p = 3, against the plain
+    forward solve of the same steps."""
+    import torch
+    mesh = vt.generate_mesh(nx=n, ny=n)
+    s = vt.Solver(mesh, 3, 8)
+    tgt = s.exact_vortex(dt * steps, 0.32, 0.1, -0.05)[:, :, 0]
+    s.initialize_hydrodinamics(0)
+    U0 = s.state()
+    s.rk4(dt, 64)
+    s.set_state(U0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.rk4(dt, steps)
+    tf = time.perf_counter() - t0
+    s.set_state(U0)
+    t0 = time.perf_counter()
+    J, lam0 = s.adjoint(dt, steps, tgt, every)
+    ta = time.perf_counter() - t0
+    s.close()
+    return {"workload": "%dx%d cells, 10x10 domain, p=3, integration_order 8, %d RK4 steps of dt %g, "
+                        "checkpoints every %d steps, J = mass-weighted L2 misfit to the advected vortex "
+                        "(eps 0.32, centre (0.1, -0.05))" % (n, n, steps, dt, every),
+            "elements": 2 * n * n, "forward_s": tf, "forward_plus_adjoint_s": ta, "ratio": ta / tf, "J": J,
+            "grad_norm": float(np.linalg.norm(lam0))}
 
 
 def cpu_baseline(mesh, p, order, steps):
@@ -171,6 +252,10 @@ def main():
                      "avg_launch_ms": avg_launch_s * 1e3, "stage_ms": [float(x) for x in stage_ms]},
     }
     s.close()
+    if not a.no_batched:
+        line["batched"] = batched_figure(vt, a.batched_steps)
+    if not a.no_adjoint:
+        line["adjoint"] = adjoint_figure(vt, a.adjoint_steps)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mesh, p, order, a.cpu_steps)
     print(json.dumps(line))

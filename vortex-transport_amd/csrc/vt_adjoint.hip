@@ -29,7 +29,9 @@ size_t slot_elems(const vt_ctx* c) { return static_cast<size_t>(4) * c->np * c->
 int ensure_adj_planes(vt_ctx* c) {
     if (c->adj) return VT_OK;
     HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->adj), sizeof(double) * 4 * slot_elems(c)));
-    HIP_TRY(hipMemset(c->adj, 0, sizeof(double) * 4 * slot_elems(c)));
+    // On the context stream: it does not synchronise with the null stream.
+    HIP_TRY(hipMemsetAsync(c->adj, 0, sizeof(double) * 4 * slot_elems(c), c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return VT_OK;
 }
 

@@ -283,8 +283,13 @@ int create_ctx(int n, const double* vertices, const int* neighbors, const int* t
     if (!hip_ok(hipMemcpy(c->vtx, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice), "upload vtx") ||
         !hip_ok(hipMemcpy(c->nbr, hn.data(), sizeof(int) * hn.size(), hipMemcpyHostToDevice), "upload nbr") ||
         !hip_ok(hipMemcpy(c->tab, tab.data.data(), sizeof(double) * tab.data.size(), hipMemcpyHostToDevice), "upload tab") ||
-        !hip_ok(hipMemset(c->Un, 0, sbytes), "memset") || !hip_ok(hipMemset(c->Ua, 0, sbytes), "memset") ||
-        !hip_ok(hipMemset(c->Ub, 0, sbytes), "memset") || !hip_ok(hipMemset(c->acc, 0, sbytes), "memset"))
+        // Stream-ordered clears: the context stream does not synchronise with the
+        // null stream, so a plain hipMemset could land after later uploads.
+        !hip_ok(hipMemsetAsync(c->Un, 0, sbytes, c->stream), "memset") ||
+        !hip_ok(hipMemsetAsync(c->Ua, 0, sbytes, c->stream), "memset") ||
+        !hip_ok(hipMemsetAsync(c->Ub, 0, sbytes, c->stream), "memset") ||
+        !hip_ok(hipMemsetAsync(c->acc, 0, sbytes, c->stream), "memset") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
         return fail(VT_ERR_HIP);
     *out = c;
     return VT_OK;
