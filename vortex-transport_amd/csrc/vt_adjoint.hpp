@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(256) adjoint_seed_kernel(AdjParams prm) {
     if (e >= prm.ne) return;
     const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
     const unsigned oe = static_cast<unsigned>(e) * 8u;
-    const double* __restrict__ T = prm.tab;
+    const ctab_t T = const_table(prm.tab);
     const double x0 = plane_at(prm.vtx, 0, ld8, oe), y0 = plane_at(prm.vtx, 1, ld8, oe);
     const double x1 = plane_at(prm.vtx, 2, ld8, oe), y1 = plane_at(prm.vtx, 3, ld8, oe);
     const double x2 = plane_at(prm.vtx, 4, ld8, oe), y2 = plane_at(prm.vtx, 5, ld8, oe);
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(kAdjBlock, VT_ADJ_MIN_WAVES) adjoint_stage_ker
     const bool valid = e < prm.ne;
     const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
     const unsigned oe = static_cast<unsigned>(e) * 8u;
-    const double* __restrict__ T = prm.tab;
+    const ctab_t T = const_table(prm.tab);
     const double* __restrict__ U = prm.U;
     const double* __restrict__ Y = prm.Y;
     double* my = sy + threadIdx.x;
@@ -266,9 +266,9 @@ __global__ void __launch_bounds__(kAdjBlock, VT_ADJ_MIN_WAVES) adjoint_stage_ker
         const double dx = vx[(s + 1) % 3] - vx[s], dy = vy[(s + 1) % 3] - vy[s];
         const double len = sqrt(dx * dx + dy * dy);
         const double nxv = dy / len, nyv = -dx / len;
-        const double* __restrict__ tp = T + tab_tp(P) + s * NQ * NE;
-        const double* __restrict__ tm = T + tab_tm(P, NQ) + s * NQ * NE;
-        const double* __restrict__ tl = T + tab_lift(P, NQ) + s * NQ * NE;
+        const ctab_t tp = T + tab_tp(P) + s * NQ * NE;
+        const ctab_t tm = T + tab_tm(P, NQ) + s * NQ * NE;
+        const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;
         double yn[NE][4], un[NE][4], uo[NE][4];
         const bool inblk = static_cast<unsigned>(loc[s]) < static_cast<unsigned>(BS);
 #pragma unroll

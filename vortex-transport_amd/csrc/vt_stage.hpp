@@ -119,6 +119,12 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
     return code;
 }
 
+// Operator tables are read through the constant address space: no kernel
+// writes them, and reads at wavefront-uniform offsets then compile to scalar
+// (SMEM) loads instead of per-lane vector loads with 64-bit address math.
+typedef const __attribute__((address_space(4))) double* ctab_t;
+__device__ __forceinline__ ctab_t const_table(const double* p) { return (ctab_t)p; }
+
 __device__ __forceinline__ void report(unsigned long long* err, int stage, unsigned e, int q, int s, int code) {
     const unsigned long long key = (static_cast<unsigned long long>(stage) << 40) |
                                    ((((static_cast<unsigned long long>(e) * 16 + q) * 4 + s) * 4) + code);
@@ -154,7 +160,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     const bool valid = e < prm.ne;
     const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
     const unsigned oe = static_cast<unsigned>(e) * 8u;
-    const double* __restrict__ T = prm.tab;
+    const ctab_t T = const_table(prm.tab);
     const double* __restrict__ Uin = prm.Uin;
     double* my = su + threadIdx.x;
 
@@ -240,9 +246,9 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         const double dx = vx[(s + 1) % 3] - vx[s], dy = vy[(s + 1) % 3] - vy[s];
         const double len = sqrt(dx * dx + dy * dy);
         const double nxv = dy / len, nyv = -dx / len;
-        const double* __restrict__ tp = T + tab_tp(P) + s * NQ * NE;
-        const double* __restrict__ tm = T + tab_tm(P, NQ) + s * NQ * NE;
-        const double* __restrict__ tl = T + tab_lift(P, NQ) + s * NQ * NE;
+        const ctab_t tp = T + tab_tp(P) + s * NQ * NE;
+        const ctab_t tm = T + tab_tm(P, NQ) + s * NQ * NE;
+        const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;
 #pragma unroll VT_Q_UNROLL
         for (int q = 0; q < NQ; ++q) {
             double UL[4], UR[4], F[4];
