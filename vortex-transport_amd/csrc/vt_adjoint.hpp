@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) adjoint_seed_kernel(AdjParams prm) {
         for (int j = 0; j < NP; ++j) {
             double a = 0.0;
 #pragma unroll
-            for (int i = 0; i < NP; ++i) a += T[tab_minv(P) + i * NP + j] * w[i];
+            for (int i = 0; i < NP; ++i) a += T[tab_minvt(P) + j * NP + i] * w[i];
             plane_at(prm.Yout, k * NP + j, ld8, oe) = s * a;
         }
     }
@@ -246,8 +246,8 @@ __global__ void __launch_bounds__(kAdjBlock, VT_ADJ_MIN_WAVES) adjoint_stage_ker
 #pragma unroll
             for (int i = 0; i < NP; ++i) {
                 const double yv = my[(k * NP + i) * BS];
-                zx += T[tab_sxi(P) + i * NP + j] * yv;
-                zy += T[tab_seta(P) + i * NP + j] * yv;
+                zx += T[tab_s(P) + (j * NP + i) * 2] * yv;
+                zy += T[tab_s(P) + (j * NP + i) * 2 + 1] * yv;
             }
             bFx[k] = a00 * zx + a10 * zy;
             bFy[k] = a01 * zx + a11 * zy;
@@ -316,40 +316,42 @@ __global__ void __launch_bounds__(kAdjBlock, VT_ADJ_MIN_WAVES) adjoint_stage_ker
             for (int k = 0; k < 4; ++k) plane_at(prm.Yout, k * NP + i, ld8, oe) = g[i][k];
         return;
     }
-    // Reverse-RK4 bookkeeping, one variable at a time.
+    // Reverse-RK4 bookkeeping: the next cotangent w for all variables first
+    // (g dies as w is formed), then y = Minv_ref^T w / detJ column by column.
     const double dt = prm.dt;
     const double idet = 1.0 / ((x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0));
+    double w[NP][4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        double w[NP];
+    for (int i = 0; i < NP; ++i)
 #pragma unroll
-        for (int i = 0; i < NP; ++i) {
+        for (int k = 0; k < 4; ++k) {
             const int v = k * NP + i;
             const double nu = dt * g[i][k];
             const double lam = plane_at(prm.lam, v, ld8, oe);
             if (mode == kAdj4) {
                 plane_at(prm.acc, v, ld8, oe) = lam + nu;
-                w[i] = lam * (1.0 / 3.0) + nu;
+                w[i][k] = lam * (1.0 / 3.0) + nu;
             } else if (mode == kAdj3) {
                 plane_at(prm.acc, v, ld8, oe) = plane_at(prm.acc, v, ld8, oe) + nu;
-                w[i] = lam * (1.0 / 3.0) + 0.5 * nu;
+                w[i][k] = lam * (1.0 / 3.0) + 0.5 * nu;
             } else if (mode == kAdj2) {
                 plane_at(prm.acc, v, ld8, oe) = plane_at(prm.acc, v, ld8, oe) + nu;
-                w[i] = lam * (1.0 / 6.0) + 0.5 * nu;
+                w[i][k] = lam * (1.0 / 6.0) + 0.5 * nu;
             } else {  // kAdj1: lam_n = acc + nu1, then the next step's first cotangent lam_n / 6
                 const double ln = plane_at(prm.acc, v, ld8, oe) + nu;
                 plane_at(prm.lam, v, ld8, oe) = ln;
-                w[i] = ln * (1.0 / 6.0);
+                w[i][k] = ln * (1.0 / 6.0);
             }
         }
 #pragma unroll
-        for (int j = 0; j < NP; ++j) {
+    for (int j = 0; j < NP; ++j)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
             double a = 0.0;
 #pragma unroll
-            for (int i = 0; i < NP; ++i) a += T[tab_minv(P) + i * NP + j] * w[i];
+            for (int i = 0; i < NP; ++i) a += T[tab_minvt(P) + j * NP + i] * w[i][k];
             plane_at(prm.Yout, k * NP + j, ld8, oe) = idet * a;
         }
-    }
 }
 
 template <int NQ, int P>

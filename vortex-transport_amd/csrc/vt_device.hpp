@@ -3,16 +3,19 @@
 
 namespace vt {
 
-// Packed operator table for one (P, NQ), all doubles:
-//   Sxi[NP][NP]  Seta[NP][NP]  Minv[NP][NP]        reference operators
+// Packed operator table for one (P, NQ), all doubles, laid out so that each
+// kernel loop reads one contiguous run per iteration (scalar loads):
+//   S[NP][NP][2]      S[j][i] = {Sxi[i][j], Seta[i][j]}   stiffness, column-major pairs
+//   Minv[NP][NP]      Minv_ref, row-major                 (forward: rows)
+//   MinvT[NP][NP]     Minv_ref transposed                 (adjoint: columns)
 //   tp[3][NQ][P+1]    own-side trace weights  phi_{edge node m}(plus point q)
 //   tm[3][NQ][P+1]    neighbour trace weights phi_{edge node m}(minus point q)
 //   lift[3][NQ][P+1]  tp * w_q
 constexpr int tab_np(int p) { return (p + 1) * (p + 2) / 2; }
-constexpr int tab_sxi(int) { return 0; }
-constexpr int tab_seta(int p) { return tab_np(p) * tab_np(p); }
+constexpr int tab_s(int) { return 0; }
 constexpr int tab_minv(int p) { return 2 * tab_np(p) * tab_np(p); }
-constexpr int tab_tp(int p) { return 3 * tab_np(p) * tab_np(p); }
+constexpr int tab_minvt(int p) { return 3 * tab_np(p) * tab_np(p); }
+constexpr int tab_tp(int p) { return 4 * tab_np(p) * tab_np(p); }
 constexpr int tab_tm(int p, int nq) { return tab_tp(p) + 3 * nq * (p + 1); }
 constexpr int tab_lift(int p, int nq) { return tab_tm(p, nq) + 3 * nq * (p + 1); }
 constexpr int tab_size(int p, int nq) { return tab_lift(p, nq) + 3 * nq * (p + 1); }

@@ -369,8 +369,12 @@ bool build_operator_table(int p, int nq, const double* xw, const double* Minv, c
     t.p = p; t.np = np; t.nq = nq;
     t.data.assign(tab_size(p, nq), 0.0);
     std::memcpy(t.data.data() + tab_minv(p), Minv, sizeof(double) * np * np);
-    std::memcpy(t.data.data() + tab_sxi(p), S, sizeof(double) * np * np);
-    std::memcpy(t.data.data() + tab_seta(p), S + np * np, sizeof(double) * np * np);
+    for (int i = 0; i < np; ++i)
+        for (int j = 0; j < np; ++j) {
+            t.data[tab_minvt(p) + j * np + i] = Minv[i * np + j];
+            t.data[tab_s(p) + (j * np + i) * 2] = S[i * np + j];
+            t.data[tab_s(p) + (j * np + i) * 2 + 1] = S[np * np + i * np + j];
+        }
     t.max_offedge = 0;
     for (int q = 0; q < nq; ++q) {
         const double sg = xw[2 * q], w = xw[2 * q + 1];

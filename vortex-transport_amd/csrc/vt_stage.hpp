@@ -125,6 +125,15 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
 typedef const __attribute__((address_space(4))) double* ctab_t;
 __device__ __forceinline__ ctab_t const_table(const double* p) { return (ctab_t)p; }
 
+// A zero the compiler cannot see through (a scalar register move), used to
+// give table reads distinct addresses so that they are reloaded where used
+// instead of being kept live in scalar registers across a long region.
+__device__ __forceinline__ int opaque_zero() {
+    int z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    return z;
+}
+
 __device__ __forceinline__ void report(unsigned long long* err, int stage, unsigned e, int q, int s, int code) {
     const unsigned long long key = (static_cast<unsigned long long>(stage) << 40) |
                                    ((((static_cast<unsigned long long>(e) * 16 + q) * 4 + s) * 4) + code);
@@ -225,7 +234,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
             const double gy = a10 * fx[k] + a11 * fy[k];
 #pragma unroll
             for (int i = 0; i < NP; ++i)
-                r[i][k] += T[tab_sxi(P) + i * NP + j] * gx + T[tab_seta(P) + i * NP + j] * gy;
+                r[i][k] += T[tab_s(P) + (j * NP + i) * 2] * gx + T[tab_s(P) + (j * NP + i) * 2 + 1] * gy;
         }
     }
 
@@ -295,12 +304,13 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     const double idet = 1.0 / detJ;
     const double dt = prm.dt;
     constexpr double c6 = 1.0 / 6.0;
-    auto dudt = [&](int i, int k) {
+    auto dudt_m = [&](ctab_t M, int i, int k) {
         double d = 0.0;
 #pragma unroll
-        for (int j = 0; j < NP; ++j) d += T[tab_minv(P) + i * NP + j] * r[j][k];
+        for (int j = 0; j < NP; ++j) d += M[tab_minv(P) + i * NP + j] * r[j][k];
         return d * idet;
     };
+    auto dudt = [&](int i, int k) { return dudt_m(T, i, k); };
     const int mode = prm.mode;
     if (mode == kModeRk2 || mode == kModeRk3) {
         // Two halves of the variables: the second half's loads are issued once
@@ -313,12 +323,17 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                 ac[t] = plane_at(prm.acc, 2 * h * NP + t, ld8, oe);
                 u0[t] = plane_at(prm.Un, 2 * h * NP + t, ld8, oe);
             }
+            // Node-major, so each row of Minv is read once per half and dies; the
+            // table base is made opaque per half so that its loads are not
+            // merged across the halves (that would keep all 100 entries live in
+            // scalar registers and spill them).
+            const ctab_t M = T + opaque_zero();
 #pragma unroll
-            for (int kk2 = 0; kk2 < 2; ++kk2)
+            for (int i = 0; i < NP; ++i)
 #pragma unroll
-                for (int i = 0; i < NP; ++i) {
+                for (int kk2 = 0; kk2 < 2; ++kk2) {
                     const int k = 2 * h + kk2, t = kk2 * NP + i, v = k * NP + i;
-                    const double kk = dt * dudt(i, k);
+                    const double kk = dt * dudt_m(M, i, k);
                     if (mode == kModeRk2) {  // acc = k1 + 2 k2; U3 = Un + k2/2
                         plane_at(prm.acc, v, ld8, oe) = ac[t] + 2.0 * kk;
                         plane_at(prm.Uout, v, ld8, oe) = u0[t] + kk * 0.5;
