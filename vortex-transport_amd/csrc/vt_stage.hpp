@@ -242,7 +242,10 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     // len_s * w_q * phi_i (Evolve.cpp:242-244) and subtracted (R = V - L).
     // Only the P+1 edge nodes of the element and of its neighbour enter; the
     // next side's neighbour values are requested before this side is computed.
-    const double vx[3] = {x0, x1, x2}, vy[3] = {y0, y1, y2};
+    // Only detJ survives the volume term; each side reloads its two vertices
+    // (cache hits, through an opaque offset so they are not kept live from the
+    // prologue), which keeps the surface loop free of register spills.
+    const double detJ = (x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0);
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         double nxt[NE][4];
@@ -252,7 +255,9 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
 #pragma unroll
                 for (int k = 0; k < 4; ++k) nxt[m][k] = plane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
         }
-        const double dx = vx[(s + 1) % 3] - vx[s], dy = vy[(s + 1) % 3] - vy[s];
+        const int zv = opaque_zero();
+        const double dx = plane_at(prm.vtx, 2 * ((s + 1) % 3) + zv, ld8, oe) - plane_at(prm.vtx, 2 * s + zv, ld8, oe);
+        const double dy = plane_at(prm.vtx, 2 * ((s + 1) % 3) + 1 + zv, ld8, oe) - plane_at(prm.vtx, 2 * s + 1 + zv, ld8, oe);
         const double len = sqrt(dx * dx + dy * dy);
         const double nxv = dy / len, nyv = -dx / len;
         const ctab_t tp = T + tab_tp(P) + s * NQ * NE;
@@ -300,7 +305,6 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     // dU/dt = Minv_ref R / detJ, then the stage update.  One uniform branch
     // per mode, so that the mode's loads of acc / Un are all issued before the
     // Minv products that hide their latency.
-    const double detJ = (x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0);
     const double idet = 1.0 / detJ;
     const double dt = prm.dt;
     constexpr double c6 = 1.0 / 6.0;
