@@ -19,6 +19,12 @@ __global__ void flux_kernel(const double* __restrict__ in, double* __restrict__ 
 
 #define VT_NQ_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
 
+// The launchers (and their kernels) are instantiated in vt_kernels_nq.hip,
+// one translation unit per quadrature size; none are instantiated here.
+#define X(n) extern template struct NqLaunch<n>;
+VT_NQ_CASES(X)
+#undef X
+
 hipError_t launch_stage(const StageParams& prm, int p, int nq, const LaunchConfig& cfg, hipStream_t st) {
     switch (nq) {
 #define X(n) case n: return NqLaunch<n>::stage(prm, p, cfg, st);

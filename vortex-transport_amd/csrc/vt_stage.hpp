@@ -319,13 +319,23 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     if (mode == kModeRk2 || mode == kModeRk3) {
         // Two halves of the variables: the second half's loads are issued once
         // the first half's registers are free.
+        //   stage 2: U3 = Un + k2/2 with Un = U2 - k1/2 recovered from the staged
+        //            U2 and acc = k1 (exact: Fast2Sum, |k1/2| < |Un|); neither Un
+        //            is read nor acc written.
+        //   stage 3: acc = Un + (k1 + 2 k2 + 2 k3)/6 and U4 = Un + k3, with
+        //            k2 = 2 (U3 - Un) recovered from the staged U3 (U3 - Un is
+        //            exact by Sterbenz; k2 carries U3's rounding).
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             double ac[2 * NP], u0[2 * NP];
 #pragma unroll
-            for (int t = 0; t < 2 * NP; ++t) {
-                ac[t] = plane_at(prm.acc, 2 * h * NP + t, ld8, oe);
-                u0[t] = plane_at(prm.Un, 2 * h * NP + t, ld8, oe);
+            for (int t = 0; t < 2 * NP; ++t) ac[t] = plane_at(prm.acc, 2 * h * NP + t, ld8, oe);
+            if (mode == kModeRk2) {
+#pragma unroll
+                for (int t = 0; t < 2 * NP; ++t) u0[t] = my[(2 * h * NP + t) * BS] - ac[t] * 0.5;
+            } else {
+#pragma unroll
+                for (int t = 0; t < 2 * NP; ++t) u0[t] = plane_at(prm.Un, 2 * h * NP + t, ld8, oe);
             }
             // Node-major, so each row of Minv is read once per half and dies; the
             // table base is made opaque per half so that its loads are not
@@ -338,11 +348,11 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                 for (int kk2 = 0; kk2 < 2; ++kk2) {
                     const int k = 2 * h + kk2, t = kk2 * NP + i, v = k * NP + i;
                     const double kk = dt * dudt_m(M, i, k);
-                    if (mode == kModeRk2) {  // acc = k1 + 2 k2; U3 = Un + k2/2
-                        plane_at(prm.acc, v, ld8, oe) = ac[t] + 2.0 * kk;
+                    if (mode == kModeRk2) {
                         plane_at(prm.Uout, v, ld8, oe) = u0[t] + kk * 0.5;
-                    } else {  // acc = Un + (k1 + 2 k2 + 2 k3)/6; U4 = Un + k3
-                        plane_at(prm.acc, v, ld8, oe) = u0[t] + c6 * (ac[t] + 2.0 * kk);
+                    } else {
+                        const double k2 = 2.0 * (my[v * BS] - u0[t]);
+                        plane_at(prm.acc, v, ld8, oe) = u0[t] + c6 * ((ac[t] + 2.0 * k2) + 2.0 * kk);
                         plane_at(prm.Uout, v, ld8, oe) = u0[t] + kk;
                     }
                 }
