@@ -302,7 +302,10 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         }
     }
 
-    // dU/dt = Minv_ref R / detJ, then the stage update.  One uniform branch
+    // dU/dt = Minv_ref R / detJ, then the stage update.  The update's loads and
+    // stores (acc, Un, the stage output) are nontemporal: each value is touched
+    // once by this kernel, and streaming them past the caches keeps the cached
+    // neighbour traces of the other tiles resident.  One uniform branch
     // per mode, so that the mode's loads of acc / Un are all issued before the
     // Minv products that hide their latency.
     const double idet = 1.0 / detJ;
@@ -329,13 +332,13 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         for (int h = 0; h < 2; ++h) {
             double ac[2 * NP], u0[2 * NP];
 #pragma unroll
-            for (int t = 0; t < 2 * NP; ++t) ac[t] = plane_at(prm.acc, 2 * h * NP + t, ld8, oe);
+            for (int t = 0; t < 2 * NP; ++t) ac[t] = __builtin_nontemporal_load(&plane_at(prm.acc, 2 * h * NP + t, ld8, oe));
             if (mode == kModeRk2) {
 #pragma unroll
                 for (int t = 0; t < 2 * NP; ++t) u0[t] = my[(2 * h * NP + t) * BS] - ac[t] * 0.5;
             } else {
 #pragma unroll
-                for (int t = 0; t < 2 * NP; ++t) u0[t] = plane_at(prm.Un, 2 * h * NP + t, ld8, oe);
+                for (int t = 0; t < 2 * NP; ++t) u0[t] = __builtin_nontemporal_load(&plane_at(prm.Un, 2 * h * NP + t, ld8, oe));
             }
             // Node-major, so each row of Minv is read once per half and dies; the
             // table base is made opaque per half so that its loads are not
@@ -349,11 +352,11 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                     const int k = 2 * h + kk2, t = kk2 * NP + i, v = k * NP + i;
                     const double kk = dt * dudt_m(M, i, k);
                     if (mode == kModeRk2) {
-                        plane_at(prm.Uout, v, ld8, oe) = u0[t] + kk * 0.5;
+                        __builtin_nontemporal_store(u0[t] + kk * 0.5, &plane_at(prm.Uout, v, ld8, oe));
                     } else {
                         const double k2 = 2.0 * (my[v * BS] - u0[t]);
-                        plane_at(prm.acc, v, ld8, oe) = u0[t] + c6 * ((ac[t] + 2.0 * k2) + 2.0 * kk);
-                        plane_at(prm.Uout, v, ld8, oe) = u0[t] + kk;
+                        __builtin_nontemporal_store(u0[t] + c6 * ((ac[t] + 2.0 * k2) + 2.0 * kk), &plane_at(prm.acc, v, ld8, oe));
+                        __builtin_nontemporal_store(u0[t] + kk, &plane_at(prm.Uout, v, ld8, oe));
                     }
                 }
             __builtin_amdgcn_sched_barrier(0);
@@ -361,13 +364,13 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     } else if (mode == kModeRk4) {  // U_{n+1} = acc + k4/6
         double ac[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) ac[v] = plane_at(prm.acc, v, ld8, oe);
+        for (int v = 0; v < NV; ++v) ac[v] = __builtin_nontemporal_load(&plane_at(prm.acc, v, ld8, oe));
 #pragma unroll
         for (int i = 0; i < NP; ++i)
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int v = k * NP + i;
-                plane_at(prm.Uout, v, ld8, oe) = ac[v] + c6 * (dt * dudt(i, k));
+                __builtin_nontemporal_store(ac[v] + c6 * (dt * dudt(i, k)), &plane_at(prm.Uout, v, ld8, oe));
             }
     } else if (mode == kModeRk1) {  // k1 = dt f(Un); acc = k1; U2 = Un + k1/2
 #pragma unroll
@@ -376,8 +379,8 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
             for (int k = 0; k < 4; ++k) {
                 const int v = k * NP + i;
                 const double kk = dt * dudt(i, k);
-                plane_at(prm.acc, v, ld8, oe) = kk;
-                plane_at(prm.Uout, v, ld8, oe) = my[v * BS] + kk * 0.5;
+                __builtin_nontemporal_store(kk, &plane_at(prm.acc, v, ld8, oe));
+                __builtin_nontemporal_store(my[v * BS] + kk * 0.5, &plane_at(prm.Uout, v, ld8, oe));
             }
     } else if (mode == kModeEuler) {  // U + dt dU/dt
 #pragma unroll
@@ -385,13 +388,13 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int v = k * NP + i;
-                plane_at(prm.Uout, v, ld8, oe) = my[v * BS] + dt * dudt(i, k);
+                __builtin_nontemporal_store(my[v * BS] + dt * dudt(i, k), &plane_at(prm.Uout, v, ld8, oe));
             }
     } else {  // kModeRhs
 #pragma unroll
         for (int i = 0; i < NP; ++i)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) plane_at(prm.Uout, k * NP + i, ld8, oe) = dudt(i, k);
+            for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(dudt(i, k), &plane_at(prm.Uout, k * NP + i, ld8, oe));
     }
 }
 
