@@ -318,31 +318,58 @@ __global__ void __launch_bounds__(kAdjBlock, VT_ADJ_MIN_WAVES) adjoint_stage_ker
     }
     // Reverse-RK4 bookkeeping: the next cotangent w for all variables first
     // (g dies as w is formed), then y = Minv_ref^T w / detJ column by column.
+    // Each mode loads all of its operands before its first store, so the loads
+    // are in flight together (a store may alias a later load otherwise).
     const double dt = prm.dt;
     const double idet = 1.0 / ((x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0));
     double w[NP][4];
+    if (mode == kAdj4) {
+        double lam[NP][4];
 #pragma unroll
-    for (int i = 0; i < NP; ++i)
+        for (int i = 0; i < NP; ++i)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int v = k * NP + i;
-            const double nu = dt * g[i][k];
-            const double lam = plane_at(prm.lam, v, ld8, oe);
-            if (mode == kAdj4) {
-                plane_at(prm.acc, v, ld8, oe) = lam + nu;
-                w[i][k] = lam * (1.0 / 3.0) + nu;
-            } else if (mode == kAdj3) {
-                plane_at(prm.acc, v, ld8, oe) = plane_at(prm.acc, v, ld8, oe) + nu;
-                w[i][k] = lam * (1.0 / 3.0) + 0.5 * nu;
-            } else if (mode == kAdj2) {
-                plane_at(prm.acc, v, ld8, oe) = plane_at(prm.acc, v, ld8, oe) + nu;
-                w[i][k] = lam * (1.0 / 6.0) + 0.5 * nu;
-            } else {  // kAdj1: lam_n = acc + nu1, then the next step's first cotangent lam_n / 6
-                const double ln = plane_at(prm.acc, v, ld8, oe) + nu;
-                plane_at(prm.lam, v, ld8, oe) = ln;
+            for (int k = 0; k < 4; ++k) lam[i][k] = plane_at(prm.lam, k * NP + i, ld8, oe);
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double nu = dt * g[i][k];
+                plane_at(prm.acc, k * NP + i, ld8, oe) = lam[i][k] + nu;
+                w[i][k] = lam[i][k] * (1.0 / 3.0) + nu;
+            }
+    } else if (mode == kAdj1) {  // lam_n = acc + nu1, then the next step's first cotangent lam_n / 6
+        double a[NP][4];
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) a[i][k] = plane_at(prm.acc, k * NP + i, ld8, oe);
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double ln = a[i][k] + dt * g[i][k];
+                plane_at(prm.lam, k * NP + i, ld8, oe) = ln;
                 w[i][k] = ln * (1.0 / 6.0);
             }
-        }
+    } else {  // kAdj3, kAdj2
+        const double cl = mode == kAdj3 ? (1.0 / 3.0) : (1.0 / 6.0);
+        double lam[NP][4], a[NP][4];
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                lam[i][k] = plane_at(prm.lam, k * NP + i, ld8, oe);
+                a[i][k] = plane_at(prm.acc, k * NP + i, ld8, oe);
+            }
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double nu = dt * g[i][k];
+                plane_at(prm.acc, k * NP + i, ld8, oe) = a[i][k] + nu;
+                w[i][k] = lam[i][k] * cl + 0.5 * nu;
+            }
+    }
 #pragma unroll
     for (int j = 0; j < NP; ++j)
 #pragma unroll
