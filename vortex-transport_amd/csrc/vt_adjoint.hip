@@ -47,19 +47,43 @@ vt::AdjParams adj_params(vt_ctx* c, int mode) {
     return a;
 }
 
-int upload_planes(vt_ctx* c, const double* U, double* dev) {
-    std::vector<double> h(slot_elems(c), 0.0);
-    vti::to_planes(c, U, 0, c->ne, h.data());
-    HIP_TRY(hipMemcpyAsync(dev, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return VT_OK;
-}
+int upload_planes(vt_ctx* c, const double* U, double* dev) { return vti::upload_state(c, U, dev); }
 
-int download_planes(vt_ctx* c, const double* dev, double* U) {
-    std::vector<double> h(slot_elems(c));
-    HIP_TRY(hipMemcpyAsync(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+int download_planes(vt_ctx* c, const double* dev, double* U) { return vti::download_state(c, dev, U); }
+
+// The objective on the device: J (block partial sums added on the host in
+// block order) and lam = dJ/dU written into the plane set lam.
+int device_objective(vt_ctx* c, const double* rho_target, double* J, double* lam) {
+    if (c->order < 0) return VT_ERR_QUAD;  // M_ref comes from the integration order's area rule
+    const int nblk = (c->ne + vt::kObjBlock - 1) / vt::kObjBlock;
+    if (!c->mref) {
+        std::vector<double> M(static_cast<size_t>(c->np) * c->np);
+        if (!vt::reference_operators(c->p, c->order, M.data(), nullptr, nullptr)) return VT_ERR_QUAD;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->mref), sizeof(double) * M.size()));
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->tgt), sizeof(double) * c->np * c->ld));
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->part), sizeof(double) * nblk));
+        HIP_TRY(hipMemcpyAsync(c->mref, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    double* h = vti::host_stage(c, static_cast<size_t>(c->np) * c->ld);
+    if (!h) return VT_ERR_HIP;
+    vt::density_to_planes(c->ne, c->np, c->ld, rho_target, h);
+    HIP_TRY(hipMemcpyAsync(c->tgt, h, sizeof(double) * c->np * c->ld, hipMemcpyHostToDevice, c->stream));
+    vt::ObjParams o{};
+    o.U = c->Un;
+    o.tgt = c->tgt;
+    o.mref = c->mref;
+    o.vtx = c->vtx;
+    o.lam = lam;
+    o.part = c->part;
+    o.ne = c->ne;
+    o.ld = c->ld;
+    HIP_TRY(vt::launch_objective(o, c->p, c->stream));
+    HIP_TRY(hipMemcpyAsync(h, c->part, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    vti::from_planes(c, h.data(), 0, c->ne, U);
+    double sum = 0.0;
+    for (int b = 0; b < nblk; ++b) sum += h[b];
+    *J = sum;
     return VT_OK;
 }
 
@@ -89,6 +113,49 @@ hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* 
     return hipSuccess;
 }
 
+// The reverse sweep over the recorded run, from lam = dJ/dU_T in the first
+// adjoint plane set to lam = dJ/dU_0 in the same set.
+int backward_sweep(vt_ctx* c) {
+    const int every = c->rec_every, nsteps = c->rec_steps, nseg = (nsteps + every - 1) / every;
+    const double dt = c->rec_dt;
+    const size_t n = slot_elems(c);
+    const int need = 4 * (nsteps < every ? nsteps : every) + 1;
+    if (c->seg_slots < need) {
+        if (c->seg) (void)hipFree(c->seg);
+        c->seg = nullptr;
+        c->seg_slots = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->seg), sizeof(double) * n * need));
+        c->seg_slots = need;
+    }
+    double *lam = c->adj, *acc = c->adj + n, *y[2] = {c->adj + 2 * n, c->adj + 3 * n};
+    vt::AdjParams a = adj_params(c, vt::kAdjSeed);
+    a.lam = lam;
+    a.Yout = y[0];
+    a.scale = 1.0 / 6.0;
+    HIP_TRY(vt::launch_adjoint(a, c->p, c->nq, c->stream));
+    int cur = 0;
+    auto slot = [&](int step, int stage) { return c->seg + (static_cast<size_t>(4) * step + stage) * n; };
+    for (int i = nseg - 1; i >= 0; --i) {
+        const int len = (i + 1 < nseg) ? every : nsteps - i * every;
+        HIP_TRY(hipMemcpyAsync(slot(0, 0), c->ckpt[i], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        for (int s = 0; s < len; ++s)
+            HIP_TRY(recompute_step(c, dt, slot(s, 0), slot(s, 1), slot(s, 2), slot(s, 3), slot(s + 1, 0)));
+        for (int s = len - 1; s >= 0; --s)
+            for (int st = 3; st >= 0; --st) {
+                vt::AdjParams b = adj_params(c, vt::kAdj1 + st);
+                b.U = slot(s, st);
+                b.Y = y[cur];
+                b.Yout = y[cur ^ 1];
+                b.lam = lam;
+                b.acc = acc;
+                b.dt = dt;
+                HIP_TRY(vt::launch_adjoint(b, c->p, c->nq, c->stream));
+                cur ^= 1;
+            }
+    }
+    return VT_OK;
+}
+
 }  // namespace
 
 void vti::free_adjoint(vt_ctx* c) {
@@ -97,8 +164,11 @@ void vti::free_adjoint(vt_ctx* c) {
     c->ckpt.clear();
     if (c->seg) (void)hipFree(c->seg);
     if (c->adj) (void)hipFree(c->adj);
+    for (double* p : {c->mref, c->tgt, c->part})
+        if (p) (void)hipFree(p);
     c->seg = nullptr;
     c->adj = nullptr;
+    c->mref = c->tgt = c->part = nullptr;
     c->seg_slots = 0;
     c->recorded = false;
 }
@@ -159,46 +229,11 @@ int vt_adjoint_backward(vt_ctx* c, const double* lamT, double* lam0) {
     if (!c->recorded) return VT_ERR_ARG;
     int rc = ensure_adj_planes(c);
     if (rc) return rc;
-    const int every = c->rec_every, nsteps = c->rec_steps, nseg = (nsteps + every - 1) / every;
-    const double dt = c->rec_dt;
-    const size_t n = slot_elems(c);
-    const int need = 4 * (nsteps < every ? nsteps : every) + 1;
-    if (c->seg_slots < need) {
-        if (c->seg) (void)hipFree(c->seg);
-        c->seg = nullptr;
-        c->seg_slots = 0;
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->seg), sizeof(double) * n * need));
-        c->seg_slots = need;
-    }
-    double *lam = c->adj, *acc = c->adj + n, *y[2] = {c->adj + 2 * n, c->adj + 3 * n};
-    rc = upload_planes(c, lamT, lam);
+    rc = upload_planes(c, lamT, c->adj);
     if (rc) return rc;
-    vt::AdjParams a = adj_params(c, vt::kAdjSeed);
-    a.lam = lam;
-    a.Yout = y[0];
-    a.scale = 1.0 / 6.0;
-    HIP_TRY(vt::launch_adjoint(a, c->p, c->nq, c->stream));
-    int cur = 0;
-    auto slot = [&](int step, int stage) { return c->seg + (static_cast<size_t>(4) * step + stage) * n; };
-    for (int i = nseg - 1; i >= 0; --i) {
-        const int len = (i + 1 < nseg) ? every : nsteps - i * every;
-        HIP_TRY(hipMemcpyAsync(slot(0, 0), c->ckpt[i], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-        for (int s = 0; s < len; ++s)
-            HIP_TRY(recompute_step(c, dt, slot(s, 0), slot(s, 1), slot(s, 2), slot(s, 3), slot(s + 1, 0)));
-        for (int s = len - 1; s >= 0; --s)
-            for (int st = 3; st >= 0; --st) {
-                vt::AdjParams b = adj_params(c, vt::kAdj1 + st);
-                b.U = slot(s, st);
-                b.Y = y[cur];
-                b.Yout = y[cur ^ 1];
-                b.lam = lam;
-                b.acc = acc;
-                b.dt = dt;
-                HIP_TRY(vt::launch_adjoint(b, c->p, c->nq, c->stream));
-                cur ^= 1;
-            }
-    }
-    return download_planes(c, lam, lam0);
+    rc = backward_sweep(c);
+    if (rc) return rc;
+    return download_planes(c, c->adj, lam0);
 }
 
 int vt_vortex_tangent(const vt_ctx* c, double t, double epsilon, double x0, double y0, double* dU) {
@@ -209,23 +244,25 @@ int vt_vortex_tangent(const vt_ctx* c, double t, double epsilon, double x0, doub
 
 int vt_mass_l2_objective(vt_ctx* c, const double* rho_target, double* J, double* dJdU) {
     if (!c || !rho_target || !J) return VT_ERR_ARG;
-    if (c->order < 0) return VT_ERR_QUAD;
-    std::vector<double> M(static_cast<size_t>(c->np) * c->np), U(static_cast<size_t>(c->ne) * c->np * 4);
-    if (!vt::reference_operators(c->p, c->order, M.data(), nullptr, nullptr)) return VT_ERR_QUAD;
-    int rc = vt_get_state(c, U.data());
+    int rc = ensure_adj_planes(c);
     if (rc) return rc;
-    *J = vt::mass_l2_objective(c->ne, c->vertices.data(), c->p, M.data(), U.data(), rho_target, dJdU);
-    return VT_OK;
+    rc = device_objective(c, rho_target, J, c->adj);
+    if (rc || !dJdU) return rc;
+    return download_planes(c, c->adj, dJdU);
 }
 
 int vt_adjoint(vt_ctx* c, double dt, int nsteps, int every, const double* rho_target, double* J, double* dJdU0) {
     if (!c || !rho_target || !J || !dJdU0) return VT_ERR_ARG;
     int rc = vt_adjoint_forward(c, dt, nsteps, every);
     if (rc) return rc;
-    std::vector<double> g(static_cast<size_t>(c->ne) * c->np * 4);
-    rc = vt_mass_l2_objective(c, rho_target, J, g.data());
+    rc = ensure_adj_planes(c);
     if (rc) return rc;
-    return vt_adjoint_backward(c, g.data(), dJdU0);
+    // The seed dJ/dU_T goes from the objective kernel straight into lam.
+    rc = device_objective(c, rho_target, J, c->adj);
+    if (rc) return rc;
+    rc = backward_sweep(c);
+    if (rc) return rc;
+    return download_planes(c, c->adj, dJdU0);
 }
 
 }  // extern "C"

@@ -20,17 +20,45 @@ bool hip_ok(hipError_t e, const char* what) {
 }
 
 void to_planes(const vt_ctx* c, const double* U, int e0, int n, double* planes) {
-    for (int e = 0; e < n; ++e)
-        for (int j = 0; j < c->np; ++j)
-            for (int k = 0; k < 4; ++k)
-                planes[static_cast<size_t>(k * c->np + j) * c->ld + e0 + e] = U[(static_cast<size_t>(e) * c->np + j) * 4 + k];
+    vt::aos_to_planes(n, c->np, c->ld, e0, U, planes);
 }
 
 void from_planes(const vt_ctx* c, const double* planes, int e0, int n, double* U) {
-    for (int e = 0; e < n; ++e)
-        for (int j = 0; j < c->np; ++j)
-            for (int k = 0; k < 4; ++k)
-                U[(static_cast<size_t>(e) * c->np + j) * 4 + k] = planes[static_cast<size_t>(k * c->np + j) * c->ld + e0 + e];
+    vt::planes_to_aos(n, c->np, c->ld, e0, planes, U);
+}
+
+double* host_stage(vt_ctx* c, size_t n) {
+    if (c->host_stage_cap >= n) return c->host_stage;
+    if (c->host_stage) (void)hipHostFree(c->host_stage);
+    c->host_stage = nullptr;
+    c->host_stage_cap = 0;
+    if (!hip_ok(hipHostMalloc(reinterpret_cast<void**>(&c->host_stage), sizeof(double) * n, hipHostMallocDefault),
+                "hipHostMalloc staging"))
+        return nullptr;
+    c->host_stage_cap = n;
+    return c->host_stage;
+}
+
+int upload_state(vt_ctx* c, const double* U, double* dev) {
+    const size_t n = static_cast<size_t>(4) * c->np * c->ld;
+    double* h = host_stage(c, n);
+    if (!h) return VT_ERR_HIP;
+    to_planes(c, U, 0, c->ne, h);
+    for (int v = 0; v < 4 * c->np; ++v)
+        std::memset(h + static_cast<size_t>(v) * c->ld + c->ne, 0, sizeof(double) * (c->ld - c->ne));
+    HIP_TRY(hipMemcpyAsync(dev, h, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
+int download_state(vt_ctx* c, const double* dev, double* U) {
+    const size_t n = static_cast<size_t>(4) * c->np * c->ld;
+    double* h = host_stage(c, n);
+    if (!h) return VT_ERR_HIP;
+    HIP_TRY(hipMemcpyAsync(h, dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    from_planes(c, h, 0, c->ne, U);
+    return VT_OK;
 }
 
 std::vector<int> replay_plan(int method, int nsteps) {
@@ -133,13 +161,7 @@ int get_graph(vt_ctx* c, int method, double dt, int k, hipGraphExec_t* out) {
     return VT_OK;
 }
 
-int read_planes(vt_ctx* c, const double* dev, double* U) {
-    std::vector<double> h(static_cast<size_t>(4) * c->np * c->ld);
-    HIP_TRY(hipMemcpyAsync(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    vti::from_planes(c, h.data(), 0, c->ne, U);
-    return VT_OK;
-}
+int read_planes(vt_ctx* c, const double* dev, double* U) { return vti::download_state(c, dev, U); }
 
 void decode_error(vt_ctx* c, unsigned long long key, int step_base, int per_step) {
     const unsigned long long stage = key >> 40;
@@ -332,6 +354,7 @@ void vt_destroy(vt_ctx* c) {
         if (b) hipFree(b);
     if (c->err_init) hipHostFree(c->err_init);
     if (c->err_host) hipHostFree(c->err_host);
+    if (c->host_stage) hipHostFree(c->host_stage);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -356,11 +379,7 @@ int vt_exact_vortex(const vt_ctx* c, double t, double epsilon, double x0, double
 
 int vt_set_state(vt_ctx* c, const double* U) {
     if (!c || !U) return VT_ERR_ARG;
-    std::vector<double> h(static_cast<size_t>(4) * c->np * c->ld, 0.0);
-    vti::to_planes(c, U, 0, c->ne, h.data());
-    HIP_TRY(hipMemcpyAsync(c->Un, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return VT_OK;
+    return vti::upload_state(c, U, c->Un);
 }
 
 int vt_get_state(vt_ctx* c, double* U) {

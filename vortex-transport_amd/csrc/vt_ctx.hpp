@@ -20,6 +20,8 @@ struct vt_ctx {
     unsigned long long* err_init = nullptr;  // pinned host, holds kNoError
     unsigned long long* err_host = nullptr;  // pinned host, one slot per replay
     size_t err_host_cap = 0;
+    double* host_stage = nullptr;            // pinned host staging for plane transfers
+    size_t host_stage_cap = 0;               // in doubles
     std::vector<double> vertices;
     double time = 0.0;
     vt::LaunchConfig cfg;
@@ -30,6 +32,9 @@ struct vt_ctx {
     std::vector<double*> ckpt;
     double* seg = nullptr;
     double* adj = nullptr;
+    double* mref = nullptr;                  // M_ref [Np][Np] of the objective
+    double* tgt = nullptr;                   // target density planes [Np][ld]
+    double* part = nullptr;                  // objective partial sums, one per block
     int seg_slots = 0;
     int rec_steps = 0, rec_every = 0;
     double rec_dt = 0.0;
@@ -56,6 +61,14 @@ int create_ctx(int n, const double* vertices, const int* neighbors, const int* t
 // [n][Np][4] state <-> planes, for elements [e0, e0 + n) of the context.
 void to_planes(const vt_ctx* c, const double* U, int e0, int n, double* planes);
 void from_planes(const vt_ctx* c, const double* planes, int e0, int n, double* U);
+
+// Pinned host staging of at least n doubles, kept by the context.
+double* host_stage(vt_ctx* c, size_t n);
+
+// Whole-state transfers through the staging buffer: [ne][Np][4] host state
+// to / from a device plane set of the context (padding columns zeroed).
+int upload_state(vt_ctx* c, const double* U, double* dev);
+int download_state(vt_ctx* c, const double* dev, double* U);
 
 // Steps per graph replay for nsteps steps (Euler replays hold an even count,
 // except a single trailing step).

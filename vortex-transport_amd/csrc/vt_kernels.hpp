@@ -78,10 +78,27 @@ struct AdjParams {
     int ne, ld, mode, blocks_per_xcd;
 };
 
+// Objective of the adjoint, J = sum_e detJ_e drho_e^T M_ref drho_e with
+// drho = rho(U) - rho_target at the nodes: one thread per element writes
+// lam = dJ/dU (density rows 2 detJ M_ref drho, momentum and energy rows 0)
+// and each block of kObjBlock elements its partial sum of J to part[block].
+constexpr int kObjBlock = 256;
+
+struct ObjParams {
+    const double* U;     // state planes
+    const double* tgt;   // target density planes [Np][ld]
+    const double* mref;  // M_ref [Np][Np], row-major
+    const double* vtx;   // vertex planes
+    double* lam;         // dJ/dU planes
+    double* part;        // partial sums of J, one per block
+    int ne, ld;
+};
+
 hipError_t launch_stage(const StageParams& prm, int p, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st);
 hipError_t launch_batched(const BatchParams& bp, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st);
+hipError_t launch_objective(const ObjParams& prm, int p, hipStream_t st);
 
 inline int tiles_per_xcd(int ne, int block) { return ((ne + block - 1) / block + 7) / 8; }
 inline int blocks_per_xcd(int ne) { return tiles_per_xcd(ne, kBlock); }

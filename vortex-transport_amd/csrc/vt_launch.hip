@@ -16,6 +16,46 @@ __global__ void flux_kernel(const double* __restrict__ in, double* __restrict__ 
     for (int k = 0; k < 4; ++k) out[4 * static_cast<size_t>(i) + k] = F[k];
 }
 
+// The objective (see ObjParams).  Per element the sum runs over nodes in order,
+// as (detJ drho_i) (M_ref drho)_i; the block sum is a fixed-shape tree, so J
+// is reproducible run to run.
+template <int P>
+__global__ void __launch_bounds__(kObjBlock) objective_kernel(ObjParams prm) {
+    constexpr int NP = (P + 1) * (P + 2) / 2;
+    __shared__ double red[kObjBlock];
+    const int e = blockIdx.x * kObjBlock + static_cast<int>(threadIdx.x);
+    double je = 0.0;
+    if (e < prm.ne) {
+        const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
+        const unsigned oe = static_cast<unsigned>(e) * 8u;
+        const ctab_t M = const_table(prm.mref);
+        const double x0 = plane_at(prm.vtx, 0, ld8, oe), y0 = plane_at(prm.vtx, 1, ld8, oe);
+        const double x1 = plane_at(prm.vtx, 2, ld8, oe), y1 = plane_at(prm.vtx, 3, ld8, oe);
+        const double x2 = plane_at(prm.vtx, 4, ld8, oe), y2 = plane_at(prm.vtx, 5, ld8, oe);
+        const double det = (x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0);
+        double d[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) d[i] = plane_at(prm.U, i, ld8, oe) - plane_at(prm.tgt, i, ld8, oe);
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) a += M[i * NP + j] * d[j];
+            je += det * d[i] * a;
+            plane_at(prm.lam, i, ld8, oe) = 2.0 * det * a;
+        }
+#pragma unroll
+        for (int v = NP; v < 4 * NP; ++v) plane_at(prm.lam, v, ld8, oe) = 0.0;
+    }
+    red[threadIdx.x] = je;
+    __syncthreads();
+#pragma unroll
+    for (int w = kObjBlock / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) prm.part[blockIdx.x] = red[0];
+}
 
 #define VT_NQ_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
 
@@ -55,6 +95,18 @@ hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st) {
 hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(flux_kernel, dim3((n + 255) / 256), dim3(256), 0, st, in, out, codes, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_objective(const ObjParams& prm, int p, hipStream_t st) {
+    const dim3 grid((prm.ne + kObjBlock - 1) / kObjBlock), block(kObjBlock);
+    switch (p) {
+    case 0: hipLaunchKernelGGL(objective_kernel<0>, grid, block, 0, st, prm); break;
+    case 1: hipLaunchKernelGGL(objective_kernel<1>, grid, block, 0, st, prm); break;
+    case 2: hipLaunchKernelGGL(objective_kernel<2>, grid, block, 0, st, prm); break;
+    case 3: hipLaunchKernelGGL(objective_kernel<3>, grid, block, 0, st, prm); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

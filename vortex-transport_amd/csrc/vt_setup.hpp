@@ -61,10 +61,12 @@ bool initial_state(int ne, const double* vertices, int p, int order, int init_ty
 // (epsilon, x0, y0): dU [3][ne][Np][4] (closed form of Element.cpp:134-166).
 void vortex_tangent(int ne, const double* vertices, int p, double t, double epsilon, double x0, double y0,
                     double* dU);
-// J = sum_e detJ_e drho_e^T M_ref drho_e, drho = U[:, :, 0] - rho_target ([ne][Np]);
-// dJdU [ne][Np][4] (may be null) receives the gradient.  M_ref: [Np][Np].
-double mass_l2_objective(int ne, const double* vertices, int p, const double* M_ref, const double* U,
-                         const double* rho_target, double* dJdU);
+// Element-major [n][Np][4] state <-> structure-of-arrays planes: value k of
+// node j of element e lives at planes[(k * Np + j) * ld + e0 + e].  A density
+// field [n][Np] goes to Np planes (j * ld + e).  Threaded over element blocks.
+void aos_to_planes(int n, int np, int ld, int e0, const double* U, double* planes);
+void planes_to_aos(int n, int np, int ld, int e0, const double* planes, double* U);
+void density_to_planes(int n, int np, int ld, const double* rho, double* planes);
 // Nodal flux from a state, the refresh of Timestepping.cpp:59-76.
 void nodal_flux(const double* U /*[4]*/, double* F /*[2][4]*/);
 
