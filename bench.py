@@ -12,9 +12,12 @@ ci = c2^2 ~ 16), after which the reference exits.  The work per step does not
 depend on dt.
 
 Beside the plain figure the line carries "batched" (the 64-instance
-ensemble, one launch per RK stage, against per-instance plain solvers) and
+ensemble, one launch per RK stage, against per-instance plain solvers),
 "adjoint" (forward + checkpointed discrete adjoint on 256 x 256 cells, against
-the forward solve alone).
+the forward solve alone), "batched_adjoint" (32 uneven instances up to
+512 x 512 cells, recorded forward + reverse sweep + per-instance parameter
+gradients, against the batched forward) and "large_mesh" (a forward-only
+1024 x 1024-cell run).
 
 Prints one JSON line (rank 0).  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -55,6 +58,10 @@ def parse_args():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=64)
     ap.add_argument("--adjoint-steps", type=int, default=1000)
+    ap.add_argument("--no-batched-adjoint", action="store_true")
+    ap.add_argument("--batched-adjoint-steps", type=int, default=500)
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=20)
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 --pmc: 2 steps of per-launch stage kernels, then a plain device copy "
                          "of 2^27 doubles (1 GiB read + 1 GiB written) to calibrate FETCH_SIZE / WRITE_SIZE")
@@ -160,6 +167,72 @@ def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
             "grad_norm": float(np.linalg.norm(lam0))}
 
 
+def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
+    """Ensemble adjoint with memory stress: 32 instances (seed 5: cells per
+    side 64..512, p = 3, epsilon 0.1..0.4, centre in [-2.5, 2.5]^2 of a
+    10 x 10 domain), the recorded forward run, per-instance objectives (misfit
+    to the advected target vortex eps 0.32, centre (0.1, -0.05)), the reverse
+    sweep and dJ_i/d(eps_i, x0_i, y0_i), all instances in one launch per
+    stage; against the batched forward of the same steps.  dt = 5e-5 keeps
+    dt / h of the 256-cell instances of the 64-instance ensemble at h = 10/512."""
+    import torch
+    draws = vt.ensemble_draws(5, 32, cells=(64, 512), orders_p=(3,))
+    meshes = [vt.generate_mesh(nx=d["cells"], ny=d["cells"]) for d in draws]
+    eps, x0, y0 = ([d[k] for d in draws] for k in ("epsilon", "x0", "y0"))
+    b = vt.Batch(meshes, 3, 8)
+    b.initialize_hydrodinamics(0, eps, x0, y0)
+    U0 = [b.state(i) for i in range(b.n)]
+    tg = [b.exact_vortex(i, dt * steps, 0.32, 0.1, -0.05)[:, :, 0] for i in range(b.n)]
+    b.rk4(dt, 4)
+    for i in range(b.n):
+        b.set_state(i, U0[i])
+    b.time = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.rk4(dt, steps)
+    tf = time.perf_counter() - t0
+    for i in range(b.n):
+        b.set_state(i, U0[i])
+    b.time = 0.0
+    t0 = time.perf_counter()
+    J, lam = b.adjoint(dt, steps, tg, every=every)
+    grad = b.vortex_gradient(0.0, eps, x0, y0, lam)
+    ta = time.perf_counter() - t0
+    ne = sum(b.ne)
+    slot_bytes = ne * 10 * 4 * 8
+    slots = -(-steps // every) + 4 * every + 9
+    b.close()
+    return {"workload": "32 instances (seed 5): cells 64..512 per side, p=3, eps 0.1..0.4, 10x10 domain, "
+                        "integration_order 8, %d RK4 steps of dt %g, checkpoints every %d steps" % (steps, dt, every),
+            "elements": ne, "steps": steps, "forward_s": tf, "forward_ms_per_step": tf / steps * 1e3,
+            "forward_plus_adjoint_s": ta, "ratio": ta / tf, "launches_per_stage": 1,
+            "device_state_gb": slots * slot_bytes / 1e9, "J_sum": float(np.sum(J)),
+            "grad_finite": bool(np.isfinite(grad).all()), "grad_abs_max": float(np.abs(grad).max())}
+
+
+def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
+    """Forward-only p = 3 run on n x n cells (2 n^2 triangles) of the 20 x 20
+    perturbed mesh (seed 2): device-resident state sizing and throughput."""
+    import torch
+    mesh = vt.generate_mesh_seeded(n, n, 20.0, 20.0, 0.02, 2)
+    s = vt.Solver(mesh, 3, 8)
+    s.initialize_hydrodinamics(0)
+    s.rk4(dt, 4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.rk4(dt, steps)
+    wall = time.perf_counter() - t0
+    free, total = torch.cuda.mem_get_info()
+    ne = s.ne
+    ok = bool(np.isfinite(s.state()).all())
+    s.close()
+    return {"workload": "%dx%d cells (%d triangles), 20x20 perturbed mesh (seed 2), p=3, integration_order 8, "
+                        "RK4, dt %g" % (n, n, ne, dt),
+            "elements": ne, "steps": steps, "ms_per_step": wall / steps * 1e3,
+            "value": ne * 10 * 4 * 4 * steps / wall, "unit": "DOF-stage updates/s",
+            "state_bytes": ne * 10 * 4 * 8, "device_used_gb": (total - free) / 1e9, "finite": ok}
+
+
 def cpu_baseline(mesh, p, order, steps):
     """The reference's own translation units (oracle/_ref/vt_ref_run, OpenMP)
     timed on this host over `steps` RK4 steps of the same mesh; falls back to
@@ -263,6 +336,10 @@ def main():
         line["batched"] = batched_figure(vt, a.batched_steps)
     if not a.no_adjoint:
         line["adjoint"] = adjoint_figure(vt, a.adjoint_steps)
+    if not a.no_batched_adjoint:
+        line["batched_adjoint"] = batched_adjoint_figure(vt, a.batched_adjoint_steps)
+    if not a.no_large:
+        line["large_mesh"] = large_mesh_figure(vt, a.large_steps)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mesh, p, order, a.cpu_steps)
     print(json.dumps(line))

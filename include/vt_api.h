@@ -202,6 +202,29 @@ int vt_batch_get_launch(const vt_batch *b, int *cfg /* [3] */);
  * from the current state each time, keeps the fastest and restores state
  * and time.  best_cfg [3] and best_ms (per step) may be NULL. */
 int vt_batch_autotune(vt_batch *b, double dt, int trial_steps, int *best_cfg, double *best_ms);
+/* Type 0 vortex state of instance inst at time t (vt_exact_vortex). */
+int vt_batch_exact_vortex(const vt_batch *b, int inst, double t, double epsilon, double x0, double y0, double *U);
+
+/* ---- Ensemble adjoint (no reference counterpart) ----------------------------
+ * Every instance i has its own objective J_i of the vt_mass_l2_objective form
+ * and gradient dJ_i/dU0_i.  Instances with the same (p, integration order)
+ * share every launch of the forward, objective and reverse sweep, as they
+ * share the launches of vt_batch_step.  Concatenated arrays list the instances
+ * in order: rho_target [sum n_i][Np_i], dJdU [sum n_i][Np_i][4]. */
+/* J [n_inst] and, when dJdU is not NULL, dJ_i/dU_i at the current state. */
+int vt_batch_mass_l2_objective(vt_batch *b, const double *rho_target, double *J, double *dJdU);
+/* nsteps RK4 steps of dt recorded with a checkpoint every `every` steps
+ * (the state advances to U_T as with vt_batch_step), the objectives J [n_inst]
+ * at U_T, and the reverse sweep: dJdU0 = dJ_i/dU0_i.  Device memory per
+ * (p, order) group of E elements: (ceil(nsteps/every) + 4 every + 9) states
+ * of E * Np * 32 bytes.  VT_ERR_NONPHYSICAL as vt_batch_step. */
+int vt_batch_adjoint(vt_batch *b, double dt, int nsteps, int every, const double *rho_target, double *J,
+                     double *dJdU0);
+/* grad [n_inst][3] = dJ_i/d(epsilon_i, x0_i, y0_i) of type 0 vortex initial
+ * states set at time t, from dJdU0 of vt_batch_adjoint (the tangent of
+ * vt_vortex_tangent contracted with dJ_i/dU0_i). */
+int vt_batch_vortex_gradient(const vt_batch *b, double t, const double *epsilon, const double *x0, const double *y0,
+                             const double *dJdU0, double *grad);
 
 #ifdef __cplusplus
 }

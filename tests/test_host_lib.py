@@ -154,3 +154,13 @@ def test_batch_create_rejects_bad_arguments_before_touching_the_device(vt):
     with pytest.raises(vt.VortexError) as e:
         vt.Batch([], 3, 8)
     assert e.value.status == vt.VT_ERR_ARG
+
+
+def test_batch_adjoint_entry_points_reject_null_handles(vt):
+    # argument checks only: nothing reaches the GPU
+    L = vt.lib()
+    d = np.zeros(8)
+    assert L.vt_batch_adjoint(None, 1e-3, 4, 2, vt._d(d), vt._d(d), vt._d(d)) == vt.VT_ERR_ARG
+    assert L.vt_batch_mass_l2_objective(None, vt._d(d), vt._d(d), vt._d(d)) == vt.VT_ERR_ARG
+    assert L.vt_batch_vortex_gradient(None, 0.0, vt._d(d), vt._d(d), vt._d(d), vt._d(d), vt._d(d)) == vt.VT_ERR_ARG
+    assert L.vt_batch_exact_vortex(None, 0, 0.0, 0.3, 0.0, 0.0, vt._d(d)) == vt.VT_ERR_ARG

@@ -119,6 +119,11 @@ def lib():
             "vt_batch_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "vt_batch_get_launch": (ctypes.c_int, [vp, _ip]),
             "vt_batch_autotune": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
+            "vt_batch_exact_vortex": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                     ctypes.c_double, ctypes.c_double, _dp]),
+            "vt_batch_mass_l2_objective": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+            "vt_batch_adjoint": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
+            "vt_batch_vortex_gradient": (ctypes.c_int, [vp, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -546,6 +551,60 @@ class Batch:
         ms = ctypes.c_double()
         _check(lib().vt_batch_autotune(self._h, dt, trial_steps, _i(c), ctypes.byref(ms)), self)
         return {"block": int(c[0]), "persistent": bool(c[1]), "slots_per_xcd": int(c[2])}, ms.value
+
+    # --- ensemble adjoint ---------------------------------------------------
+
+    def exact_vortex(self, i, t, epsilon=0.3, x0=0.0, y0=0.0):
+        """Type 0 vortex state [n_i][Np_i][4] of instance i at time t."""
+        i = self._inst(i)
+        U = np.zeros((self.ne[i], self.np[i], 4))
+        _check(lib().vt_batch_exact_vortex(self._h, i, t, epsilon, x0, y0, _d(U)))
+        return U
+
+    def _targets(self, rho_targets):
+        if len(rho_targets) != self.n:
+            raise ValueError("one target density per instance")
+        return np.ascontiguousarray(np.concatenate(
+            [np.asarray(r, dtype=np.float64).reshape(self.ne[i] * self.np[i]) for i, r in enumerate(rho_targets)]))
+
+    def _split(self, flat):
+        out, o = [], 0
+        for i in range(self.n):
+            k = self.ne[i] * self.np[i] * 4
+            out.append(flat[o:o + k].reshape(self.ne[i], self.np[i], 4))
+            o += k
+        return out
+
+    def mass_l2_objective(self, rho_targets):
+        """(J [n], [dJ_i/dU_i]) at the current states; rho_targets: one
+        [n_i][Np_i] density per instance."""
+        rt = self._targets(rho_targets)
+        J = np.zeros(self.n)
+        g = np.zeros(4 * rt.size)
+        _check(lib().vt_batch_mass_l2_objective(self._h, _d(rt), _d(J), _d(g)), self)
+        return J, self._split(g)
+
+    def adjoint(self, dt, nsteps, rho_targets, every=32):
+        """Recorded forward run of every instance, objectives at the end and
+        the reverse sweep: (J [n], [dJ_i/dU0_i])."""
+        rt = self._targets(rho_targets)
+        J = np.zeros(self.n)
+        g = np.zeros(4 * rt.size)
+        _check(lib().vt_batch_adjoint(self._h, dt, nsteps, every, _d(rt), _d(J), _d(g)), self)
+        return J, self._split(g)
+
+    def vortex_gradient(self, t, epsilon, x0, y0, dJdU0):
+        """dJ_i/d(epsilon_i, x0_i, y0_i) [n][3] from the per-instance dJ/dU0
+        of adjoint(), for type 0 vortex initial states set at time t."""
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (self.n,)))
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(x0, dtype=np.float64), (self.n,)))
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(y0, dtype=np.float64), (self.n,)))
+        lam = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in dJdU0]))
+        if lam.size != 4 * sum(self.ne[i] * self.np[i] for i in range(self.n)):
+            raise ValueError("dJdU0 needs one [n_i][Np_i][4] array per instance")
+        grad = np.zeros((self.n, 3))
+        _check(lib().vt_batch_vortex_gradient(self._h, t, _d(e), _d(a), _d(c), _d(lam), _d(grad)))
+        return grad
 
 
 def splitmix64(seed):

@@ -14,6 +14,7 @@
 // the four adjoint stage kernels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -26,7 +27,9 @@ using vti::hip_ok;
 
 size_t slot_elems(const vt_ctx* c) { return static_cast<size_t>(4) * c->np * c->ld; }
 
-int ensure_adj_planes(vt_ctx* c) {
+}  // namespace
+
+int vti::ensure_adj_planes(vt_ctx* c) {
     if (c->adj) return VT_OK;
     HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->adj), sizeof(double) * 4 * slot_elems(c)));
     // On the context stream: it does not synchronise with the null stream.
@@ -34,6 +37,57 @@ int ensure_adj_planes(vt_ctx* c) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     return VT_OK;
 }
+
+int vti::objective(vt_ctx* c, const double* rho_target, int nr, const int* e0, const int* n, double* J) {
+    if (c->order < 0) return VT_ERR_QUAD;  // M_ref comes from the integration order's area rule
+    int rc = ensure_adj_planes(c);
+    if (rc) return rc;
+    std::vector<int> boff(nr + 1, 0);
+    for (int r = 0; r < nr; ++r) boff[r + 1] = boff[r] + (n[r] + vt::kObjBlock - 1) / vt::kObjBlock;
+    if (!c->mref) {
+        std::vector<double> M(static_cast<size_t>(c->np) * c->np);
+        if (!vt::reference_operators(c->p, c->order, M.data(), nullptr, nullptr)) return VT_ERR_QUAD;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->mref), sizeof(double) * M.size()));
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->tgt), sizeof(double) * c->np * c->ld));
+        HIP_TRY(hipMemcpyAsync(c->mref, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (c->part_cap < boff[nr]) {
+        if (c->part) (void)hipFree(c->part);
+        c->part = nullptr;
+        c->part_cap = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->part), sizeof(double) * boff[nr]));
+        c->part_cap = boff[nr];
+    }
+    double* h = host_stage(c, std::max(static_cast<size_t>(c->np) * c->ld, static_cast<size_t>(boff[nr])));
+    if (!h) return VT_ERR_HIP;
+    vt::density_to_planes(c->ne, c->np, c->ld, rho_target, h);
+    HIP_TRY(hipMemcpyAsync(c->tgt, h, sizeof(double) * c->np * c->ld, hipMemcpyHostToDevice, c->stream));
+    // One launch per element range: the plane base pointers move to the
+    // range's first element, the plane stride stays ld.
+    for (int r = 0; r < nr; ++r) {
+        vt::ObjParams o{};
+        o.U = c->Un + e0[r];
+        o.tgt = c->tgt + e0[r];
+        o.mref = c->mref;
+        o.vtx = c->vtx + e0[r];
+        o.lam = c->adj + e0[r];
+        o.part = c->part + boff[r];
+        o.ne = n[r];
+        o.ld = c->ld;
+        HIP_TRY(vt::launch_objective(o, c->p, c->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(h, c->part, sizeof(double) * boff[nr], hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int r = 0; r < nr; ++r) {
+        double sum = 0.0;
+        for (int b = boff[r]; b < boff[r + 1]; ++b) sum += h[b];
+        J[r] = sum;
+    }
+    return VT_OK;
+}
+
+namespace {
 
 vt::AdjParams adj_params(vt_ctx* c, int mode) {
     vt::AdjParams a{};
@@ -50,42 +104,6 @@ vt::AdjParams adj_params(vt_ctx* c, int mode) {
 int upload_planes(vt_ctx* c, const double* U, double* dev) { return vti::upload_state(c, U, dev); }
 
 int download_planes(vt_ctx* c, const double* dev, double* U) { return vti::download_state(c, dev, U); }
-
-// The objective on the device: J (block partial sums added on the host in
-// block order) and lam = dJ/dU written into the plane set lam.
-int device_objective(vt_ctx* c, const double* rho_target, double* J, double* lam) {
-    if (c->order < 0) return VT_ERR_QUAD;  // M_ref comes from the integration order's area rule
-    const int nblk = (c->ne + vt::kObjBlock - 1) / vt::kObjBlock;
-    if (!c->mref) {
-        std::vector<double> M(static_cast<size_t>(c->np) * c->np);
-        if (!vt::reference_operators(c->p, c->order, M.data(), nullptr, nullptr)) return VT_ERR_QUAD;
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->mref), sizeof(double) * M.size()));
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->tgt), sizeof(double) * c->np * c->ld));
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->part), sizeof(double) * nblk));
-        HIP_TRY(hipMemcpyAsync(c->mref, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
-    double* h = vti::host_stage(c, static_cast<size_t>(c->np) * c->ld);
-    if (!h) return VT_ERR_HIP;
-    vt::density_to_planes(c->ne, c->np, c->ld, rho_target, h);
-    HIP_TRY(hipMemcpyAsync(c->tgt, h, sizeof(double) * c->np * c->ld, hipMemcpyHostToDevice, c->stream));
-    vt::ObjParams o{};
-    o.U = c->Un;
-    o.tgt = c->tgt;
-    o.mref = c->mref;
-    o.vtx = c->vtx;
-    o.lam = lam;
-    o.part = c->part;
-    o.ne = c->ne;
-    o.ld = c->ld;
-    HIP_TRY(vt::launch_objective(o, c->p, c->stream));
-    HIP_TRY(hipMemcpyAsync(h, c->part, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    double sum = 0.0;
-    for (int b = 0; b < nblk; ++b) sum += h[b];
-    *J = sum;
-    return VT_OK;
-}
 
 // Forward stage kernels of one step from slot S0 into slots S1, S2, S3 and next.
 hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* S2, double* S3, double* next) {
@@ -113,9 +131,9 @@ hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* 
     return hipSuccess;
 }
 
-// The reverse sweep over the recorded run, from lam = dJ/dU_T in the first
-// adjoint plane set to lam = dJ/dU_0 in the same set.
-int backward_sweep(vt_ctx* c) {
+}  // namespace
+
+int vti::backward_sweep(vt_ctx* c) {
     const int every = c->rec_every, nsteps = c->rec_steps, nseg = (nsteps + every - 1) / every;
     const double dt = c->rec_dt;
     const size_t n = slot_elems(c);
@@ -156,8 +174,6 @@ int backward_sweep(vt_ctx* c) {
     return VT_OK;
 }
 
-}  // namespace
-
 void vti::free_adjoint(vt_ctx* c) {
     for (double* p : c->ckpt)
         if (p) (void)hipFree(p);
@@ -169,6 +185,7 @@ void vti::free_adjoint(vt_ctx* c) {
     c->seg = nullptr;
     c->adj = nullptr;
     c->mref = c->tgt = c->part = nullptr;
+    c->part_cap = 0;
     c->seg_slots = 0;
     c->recorded = false;
 }
@@ -177,7 +194,7 @@ extern "C" {
 
 int vt_rhs_vjp(vt_ctx* c, const double* w, double* g) {
     if (!c || !w || !g) return VT_ERR_ARG;
-    int rc = ensure_adj_planes(c);
+    int rc = vti::ensure_adj_planes(c);
     if (rc) return rc;
     const size_t n = slot_elems(c);
     double *lam = c->adj, *ya = c->adj + 2 * n, *yb = c->adj + 3 * n;
@@ -227,11 +244,11 @@ int vt_adjoint_forward(vt_ctx* c, double dt, int nsteps, int every) {
 int vt_adjoint_backward(vt_ctx* c, const double* lamT, double* lam0) {
     if (!c || !lamT || !lam0) return VT_ERR_ARG;
     if (!c->recorded) return VT_ERR_ARG;
-    int rc = ensure_adj_planes(c);
+    int rc = vti::ensure_adj_planes(c);
     if (rc) return rc;
     rc = upload_planes(c, lamT, c->adj);
     if (rc) return rc;
-    rc = backward_sweep(c);
+    rc = vti::backward_sweep(c);
     if (rc) return rc;
     return download_planes(c, c->adj, lam0);
 }
@@ -244,9 +261,8 @@ int vt_vortex_tangent(const vt_ctx* c, double t, double epsilon, double x0, doub
 
 int vt_mass_l2_objective(vt_ctx* c, const double* rho_target, double* J, double* dJdU) {
     if (!c || !rho_target || !J) return VT_ERR_ARG;
-    int rc = ensure_adj_planes(c);
-    if (rc) return rc;
-    rc = device_objective(c, rho_target, J, c->adj);
+    const int e0 = 0;
+    int rc = vti::objective(c, rho_target, 1, &e0, &c->ne, J);
     if (rc || !dJdU) return rc;
     return download_planes(c, c->adj, dJdU);
 }
@@ -255,12 +271,11 @@ int vt_adjoint(vt_ctx* c, double dt, int nsteps, int every, const double* rho_ta
     if (!c || !rho_target || !J || !dJdU0) return VT_ERR_ARG;
     int rc = vt_adjoint_forward(c, dt, nsteps, every);
     if (rc) return rc;
-    rc = ensure_adj_planes(c);
-    if (rc) return rc;
     // The seed dJ/dU_T goes from the objective kernel straight into lam.
-    rc = device_objective(c, rho_target, J, c->adj);
+    const int e0 = 0;
+    rc = vti::objective(c, rho_target, 1, &e0, &c->ne, J);
     if (rc) return rc;
-    rc = backward_sweep(c);
+    rc = vti::backward_sweep(c);
     if (rc) return rc;
     return download_planes(c, c->adj, dJdU0);
 }

@@ -195,6 +195,46 @@ int run_steps(vt_batch* b, int method, double dt, int nsteps, hipEvent_t ev0, hi
 
 int check_inst(const vt_batch* b, int i) { return (b && i >= 0 && i < b->n) ? VT_OK : VT_ERR_ARG; }
 
+// First node of each instance in the concatenated [sum n_i][Np_i] arrays.
+std::vector<size_t> node_offsets(const vt_batch* b) {
+    std::vector<size_t> off(b->n + 1, 0);
+    for (int i = 0; i < b->n; ++i) off[i + 1] = off[i] + static_cast<size_t>(b->ne[i]) * vt::num_nodes(b->p[i]);
+    return off;
+}
+
+// Objective of every instance of group gi at its current state: J of each
+// member, dJ/dU of the group in its lam planes.
+int group_objective(vt_batch* b, int gi, const double* rho_target, const std::vector<size_t>& off, double* J) {
+    vt_ctx* c = b->g[gi];
+    const std::vector<int>& mem = b->members[gi];
+    std::vector<double> tg(static_cast<size_t>(c->ne) * c->np);
+    std::vector<int> e0(mem.size()), n(mem.size());
+    for (size_t k = 0; k < mem.size(); ++k) {
+        const int i = mem[k];
+        std::memcpy(&tg[static_cast<size_t>(b->eoff[i]) * c->np], rho_target + off[i],
+                    sizeof(double) * static_cast<size_t>(b->ne[i]) * c->np);
+        e0[k] = b->eoff[i];
+        n[k] = b->ne[i];
+    }
+    std::vector<double> Jg(mem.size());
+    const int rc = vti::objective(c, tg.data(), static_cast<int>(mem.size()), e0.data(), n.data(), Jg.data());
+    if (rc) return rc;
+    for (size_t k = 0; k < mem.size(); ++k) J[mem[k]] = Jg[k];
+    return VT_OK;
+}
+
+// dJ/dU of group gi (its lam planes) into the members' slices of dJdU.
+int scatter_group_gradient(vt_batch* b, int gi, const std::vector<size_t>& off, double* dJdU) {
+    vt_ctx* c = b->g[gi];
+    std::vector<double> lg(static_cast<size_t>(c->ne) * c->np * 4);
+    const int rc = vti::download_state(c, c->adj, lg.data());
+    if (rc) return rc;
+    for (int i : b->members[gi])
+        std::memcpy(dJdU + 4 * off[i], &lg[static_cast<size_t>(b->eoff[i]) * c->np * 4],
+                    sizeof(double) * static_cast<size_t>(b->ne[i]) * c->np * 4);
+    return VT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -431,6 +471,64 @@ int vt_batch_autotune(vt_batch* b, double dt, int trial_steps, int* best_cfg, do
     vt_batch_set_launch(b, win.block, win.persistent, win.slots);
     if (best_cfg) { best_cfg[0] = win.block; best_cfg[1] = win.persistent; best_cfg[2] = win.slots; }
     if (best_ms) *best_ms = best / trial_steps;
+    return VT_OK;
+}
+
+int vt_batch_exact_vortex(const vt_batch* b, int i, double t, double epsilon, double x0, double y0, double* U) {
+    if (check_inst(b, i) || !U) return VT_ERR_ARG;
+    const vt_ctx* c = b->g[b->grp[i]];
+    vt::initial_state(b->ne[i], c->vertices.data() + 6 * static_cast<size_t>(b->eoff[i]), c->p, c->order, 0, epsilon,
+                      x0, y0, t, U);
+    return VT_OK;
+}
+
+int vt_batch_mass_l2_objective(vt_batch* b, const double* rho_target, double* J, double* dJdU) {
+    if (!b || !rho_target || !J) return VT_ERR_ARG;
+    const std::vector<size_t> off = node_offsets(b);
+    for (int gi = 0; gi < static_cast<int>(b->g.size()); ++gi) {
+        int rc = group_objective(b, gi, rho_target, off, J);
+        if (!rc && dJdU) rc = scatter_group_gradient(b, gi, off, dJdU);
+        if (rc) return rc;
+    }
+    return VT_OK;
+}
+
+int vt_batch_adjoint(vt_batch* b, double dt, int nsteps, int every, const double* rho_target, double* J,
+                     double* dJdU0) {
+    if (!b || !rho_target || !J || !dJdU0 || nsteps < 1 || every < 1) return VT_ERR_ARG;
+    const std::vector<size_t> off = node_offsets(b);
+    // Group by group: each group is one mesh (the disjoint union of its
+    // instances), recorded, evaluated and swept by the single-mesh adjoint.
+    for (int gi = 0; gi < static_cast<int>(b->g.size()); ++gi) {
+        vt_ctx* c = b->g[gi];
+        int rc = vt_adjoint_forward(c, dt, nsteps, every);
+        if (rc == VT_ERR_NONPHYSICAL) {
+            const int le = c->last_err[0];
+            int inst = b->members[gi][0];
+            for (int m : b->members[gi])
+                if (le >= b->eoff[m]) inst = m;
+            b->last_err[0] = inst;
+            b->last_err[1] = le - b->eoff[inst];
+            for (int k = 1; k < 6; ++k) b->last_err[k + 1] = c->last_err[k];
+        }
+        if (!rc) rc = group_objective(b, gi, rho_target, off, J);
+        if (!rc) rc = vti::backward_sweep(c);
+        if (!rc) rc = scatter_group_gradient(b, gi, off, dJdU0);
+        if (rc) return rc;
+    }
+    for (int s = 0; s < nsteps; ++s) b->time += dt;
+    return VT_OK;
+}
+
+int vt_batch_vortex_gradient(const vt_batch* b, double t, const double* eps, const double* x0, const double* y0,
+                             const double* dJdU0, double* grad) {
+    if (!b || !eps || !x0 || !y0 || !dJdU0 || !grad) return VT_ERR_ARG;
+    const std::vector<size_t> off = node_offsets(b);
+    for (int i = 0; i < b->n; ++i) {
+        const vt_ctx* c = b->g[b->grp[i]];
+        vt::vortex_gradient(b->ne[i], c->vertices.data() + 6 * static_cast<size_t>(b->eoff[i]), c->p, t, eps[i], x0[i],
+                            y0[i], dJdU0 + 4 * off[i], grad + 3 * i);
+    }
     return VT_OK;
 }
 

@@ -35,6 +35,7 @@ struct vt_ctx {
     double* mref = nullptr;                  // M_ref [Np][Np] of the objective
     double* tgt = nullptr;                   // target density planes [Np][ld]
     double* part = nullptr;                  // objective partial sums, one per block
+    int part_cap = 0;
     int seg_slots = 0;
     int rec_steps = 0, rec_every = 0;
     double rec_dt = 0.0;
@@ -80,6 +81,19 @@ int run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0, hipE
 
 // Frees the adjoint recording of a context.
 void free_adjoint(vt_ctx* c);
+
+// The adjoint's lam / acc / y plane sets (allocated and zeroed once).
+int ensure_adj_planes(vt_ctx* c);
+
+// Objective of the adjoint over nr element ranges [e0[r], e0[r] + n[r]) of the
+// current state, rho_target [ne][Np] for the whole context: J[r] per range
+// (block partial sums added in block order) and dJ/dU of every range written
+// into the lam planes.
+int objective(vt_ctx* c, const double* rho_target, int nr, const int* e0, const int* n, double* J);
+
+// The reverse sweep over the recorded forward run, from lam = dJ/dU_T in the
+// lam planes to lam = dJ/dU_0 in the same planes.
+int backward_sweep(vt_ctx* c);
 
 // Pinned host error slots, at least n of them.
 int ensure_err_slots(unsigned long long** slots, size_t* cap, size_t n);

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "vt_device.hpp"
 #include "vt_tables.h"
@@ -451,6 +452,23 @@ void vortex_tangent(int ne, const double* V, int p, double t, double epsilon, do
             }
         }
     }
+}
+
+void vortex_gradient(int ne, const double* V, int p, double t, double epsilon, double x0, double y0,
+                     const double* lam, double* grad) {
+    constexpr int kChunk = 4096;
+    const int np = num_nodes(p);
+    std::vector<double> dU(static_cast<size_t>(3) * kChunk * np * 4);
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int e0 = 0; e0 < ne; e0 += kChunk) {
+        const int n = std::min(kChunk, ne - e0);
+        const size_t plane = static_cast<size_t>(n) * np * 4;
+        vortex_tangent(n, V + 6 * static_cast<size_t>(e0), p, t, epsilon, x0, y0, dU.data());
+        const double* l = lam + static_cast<size_t>(e0) * np * 4;
+        for (int a = 0; a < 3; ++a)
+            for (size_t i = 0; i < plane; ++i) g[a] += dU[a * plane + i] * l[i];
+    }
+    for (int a = 0; a < 3; ++a) grad[a] = g[a];
 }
 
 // Blocks of 512 elements: one block's element-major slice (160 KB at p = 3)

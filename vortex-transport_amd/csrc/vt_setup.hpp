@@ -61,6 +61,11 @@ bool initial_state(int ne, const double* vertices, int p, int order, int init_ty
 // (epsilon, x0, y0): dU [3][ne][Np][4] (closed form of Element.cpp:134-166).
 void vortex_tangent(int ne, const double* vertices, int p, double t, double epsilon, double x0, double y0,
                     double* dU);
+// grad [3] = sum over elements, nodes and components of dU/d(epsilon, x0, y0)
+// (vortex_tangent at time t) times lam [ne][Np][4], element by element in
+// order, without forming the tangent.
+void vortex_gradient(int ne, const double* vertices, int p, double t, double epsilon, double x0, double y0,
+                     const double* lam, double* grad);
 // Element-major [n][Np][4] state <-> structure-of-arrays planes: value k of
 // node j of element e lives at planes[(k * Np + j) * ld + e0 + e].  A density
 // field [n][Np] goes to Np planes (j * ld + e).  Threaded over element blocks.
