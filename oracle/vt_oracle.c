@@ -332,6 +332,11 @@ oro_solver *oro_create(int ne, const double *vertices, const int *bnd, const int
     if (ne < 1 || p < 0 || p > 3) return 0;
     int nq = oro_line_rule(integration_order, 0);
     if (nq < 0 || oro_area_rule(integration_order, 0) < 0) return 0;
+    /* An exactly singular M_ref (a one-point area rule at p >= 1) has no
+     * inverse; the reference's Eigen call does not check and steps NaN states.
+     * Such configurations are refused, as the library refuses them. */
+    double minv_probe[MAXNP * MAXNP];
+    if (!Minv_ref && oro_inverse_mass_matrix_ref(p, integration_order, minv_probe)) return 0;
     for (int e = 0; e < ne; ++e)
         for (int k = 0; k < 3; ++k)
             if (bnd[3 * e + k] < 0 || bnd[3 * e + k] >= ne) return 0;

@@ -261,3 +261,35 @@ def test_fullsize_conservation(vt, big):
     m1 = mass(s.state())
     drift = np.abs(m1 - m0) / np.abs(m0)
     assert np.all(drift < 1e-13), drift
+
+
+# Every quadrature size the library compiles (orders 0..19 map to 1..10 Gauss
+# points per side), each polynomial order: a few RK4 steps against the oracle.
+# Where the reference's run fails (low integration orders make the p >= 2 mass
+# matrix singular or the state non-physical), the GPU must fail at the same
+# place.
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+@pytest.mark.parametrize("order", list(range(20)))
+def test_every_quadrature_size_matches_oracle(vt, oracle, gpu, p, order):
+    mesh = vt.generate_mesh(nx=5, ny=4, perturb_grid=1.0)
+    if p >= 1 and order <= 1:
+        # one-point area rule: M_ref is exactly singular; both refuse it (the
+        # reference inverts it unchecked and steps NaN states)
+        with pytest.raises(vt.VortexError):
+            vt.Solver(mesh, p, order)
+        with pytest.raises(ValueError):
+            oracle.Solver(mesh.element_coordinates, mesh.elements_at_boundary, mesh.element_type, p, order)
+        return
+    s = vt.Solver(mesh, p, order)
+    o = oracle.Solver(mesh.element_coordinates, mesh.elements_at_boundary, mesh.element_type, p, order)
+    s.initialize_hydrodinamics(0, 0.3, 0.4, -0.3)
+    o.set_state(s.state())
+    rc = o.run(1, 2e-3, 6)
+    if rc != 0:
+        with pytest.raises(vt.NonPhysicalStateError) as e:
+            s.rk4(2e-3, 6)
+        assert tuple(e.value.info) == o.last_error()
+    else:
+        s.rk4(2e-3, 6)
+        assert rel(s.state(), o.state()) <= 1e-12
+    s.close()

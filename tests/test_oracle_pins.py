@@ -348,3 +348,14 @@ def test_oracle_bit_exact_against_reference(oracle, path):
     if "dUdt" in d:
         dd, rc = s.rhs()
         assert np.array_equal(dd, d["dUdt"])
+
+
+def test_singular_mass_matrix_is_refused(oracle):
+    # A one-point area rule (integration orders 0 and 1) makes M_ref rank 1 for
+    # p >= 1: exactly singular, so there is no inverse to step with.
+    v, b, t = oracle.generate_mesh(4, 4, 10.0, 10.0, 0.0)
+    for p in (1, 2, 3):
+        for order in (0, 1):
+            with pytest.raises(ValueError):
+                oracle.Solver(v, b, t, p, order)
+    oracle.Solver(v, b, t, 0, 0)  # p = 0: M_ref is 1 x 1 and regular
