@@ -217,7 +217,8 @@ int vt_batch_mass_l2_objective(vt_batch *b, const double *rho_target, double *J,
  * (the state advances to U_T as with vt_batch_step), the objectives J [n_inst]
  * at U_T, and the reverse sweep: dJdU0 = dJ_i/dU0_i.  Device memory per
  * (p, order) group of E elements: (ceil(nsteps/every) + 4 every + 9) states
- * of E * Np * 32 bytes.  VT_ERR_NONPHYSICAL as vt_batch_step. */
+ * of E * Np * 32 bytes, held only while the group is processed.
+ * VT_ERR_NONPHYSICAL as vt_batch_step. */
 int vt_batch_adjoint(vt_batch *b, double dt, int nsteps, int every, const double *rho_target, double *J,
                      double *dJdU0);
 /* grad [n_inst][3] = dJ_i/d(epsilon_i, x0_i, y0_i) of type 0 vortex initial

@@ -514,6 +514,9 @@ int vt_batch_adjoint(vt_batch* b, double dt, int nsteps, int every, const double
         if (!rc) rc = group_objective(b, gi, rho_target, off, J);
         if (!rc) rc = vti::backward_sweep(c);
         if (!rc) rc = scatter_group_gradient(b, gi, off, dJdU0);
+        // The recording (checkpoints, segment states, adjoint planes) is the
+        // bulk of device memory here; it is released group by group.
+        vti::free_adjoint(c);
         if (rc) return rc;
     }
     for (int s = 0; s < nsteps; ++s) b->time += dt;
