@@ -214,9 +214,13 @@ def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
     """Forward-only p = 3 run on n x n cells (2 n^2 triangles) of the 20 x 20
     perturbed mesh (seed 2): device-resident state sizing and throughput."""
     import torch
+    c0 = time.perf_counter()
     mesh = vt.generate_mesh_seeded(n, n, 20.0, 20.0, 0.02, 2)
+    c1 = time.perf_counter()
     s = vt.Solver(mesh, 3, 8)
+    c2 = time.perf_counter()
     s.initialize_hydrodinamics(0)
+    c3 = time.perf_counter()
     s.rk4(dt, 4)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -230,7 +234,8 @@ def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
                         "RK4, dt %g" % (n, n, ne, dt),
             "elements": ne, "steps": steps, "ms_per_step": wall / steps * 1e3,
             "value": ne * 10 * 4 * 4 * steps / wall, "unit": "DOF-stage updates/s",
-            "state_bytes": ne * 10 * 4 * 8, "device_used_gb": (total - free) / 1e9, "finite": ok}
+            "state_bytes": ne * 10 * 4 * 8, "device_used_gb": (total - free) / 1e9, "finite": ok,
+            "setup_s": {"mesh": c1 - c0, "create": c2 - c1, "initial_state": c3 - c2}}
 
 
 def cpu_baseline(mesh, p, order, steps):
