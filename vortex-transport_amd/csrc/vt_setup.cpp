@@ -456,18 +456,32 @@ void vortex_tangent(int ne, const double* V, int p, double t, double epsilon, do
 
 void vortex_gradient(int ne, const double* V, int p, double t, double epsilon, double x0, double y0,
                      const double* lam, double* grad) {
-    constexpr int kChunk = 4096;
+    // Chunks of elements in parallel, each summed in element order; the chunk
+    // sums are then added in chunk order, so the result does not depend on
+    // the thread count.
+    constexpr int kChunk = 1024;
     const int np = num_nodes(p);
-    std::vector<double> dU(static_cast<size_t>(3) * kChunk * np * 4);
-    double g[3] = {0.0, 0.0, 0.0};
-    for (int e0 = 0; e0 < ne; e0 += kChunk) {
-        const int n = std::min(kChunk, ne - e0);
-        const size_t plane = static_cast<size_t>(n) * np * 4;
-        vortex_tangent(n, V + 6 * static_cast<size_t>(e0), p, t, epsilon, x0, y0, dU.data());
-        const double* l = lam + static_cast<size_t>(e0) * np * 4;
-        for (int a = 0; a < 3; ++a)
-            for (size_t i = 0; i < plane; ++i) g[a] += dU[a * plane + i] * l[i];
+    const int nchunk = (ne + kChunk - 1) / kChunk;
+    std::vector<double> part(3 * static_cast<size_t>(nchunk), 0.0);
+#pragma omp parallel
+    {
+        std::vector<double> dU(static_cast<size_t>(3) * kChunk * np * 4);
+#pragma omp for schedule(static)
+        for (int c = 0; c < nchunk; ++c) {
+            const int e0 = c * kChunk, n = std::min(kChunk, ne - e0);
+            const size_t plane = static_cast<size_t>(n) * np * 4;
+            vortex_tangent(n, V + 6 * static_cast<size_t>(e0), p, t, epsilon, x0, y0, dU.data());
+            const double* l = lam + static_cast<size_t>(e0) * np * 4;
+            for (int a = 0; a < 3; ++a) {
+                double g = 0.0;
+                for (size_t i = 0; i < plane; ++i) g += dU[a * plane + i] * l[i];
+                part[3 * static_cast<size_t>(c) + a] = g;
+            }
+        }
     }
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < nchunk; ++c)
+        for (int a = 0; a < 3; ++a) g[a] += part[3 * static_cast<size_t>(c) + a];
     for (int a = 0; a < 3; ++a) grad[a] = g[a];
 }
 
