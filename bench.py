@@ -52,6 +52,7 @@ def parse_args():
     ap.add_argument("--cells", type=int, default=512)
     ap.add_argument("--dt", type=float, default=1e-4)
     ap.add_argument("--kernel-steps", type=int, default=64, help="RK4 steps replayed for the per-launch duration")
+    ap.add_argument("--block", type=int, default=0, help="elements per workgroup of the stage kernel (0: library default)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -282,8 +283,11 @@ def main():
     p, order, n = 3, 8, a.cells
     mesh = vt.generate_mesh_seeded(n, n, 20.0, 20.0, 0.02, 2)
     s = vt.Solver(mesh, p, order)
+    if a.block:
+        s.set_launch(a.block)
     s.initialize_hydrodinamics(0)
     ne, npn = s.ne, s.np
+    block = s.launch()["block"]
     if a.profile_pass:
         s.rk4(a.dt, 1)
         profile_pass(s, a.dt)
@@ -331,7 +335,7 @@ def main():
                    "dof_stage_per_step": dof_stage},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "vt::stage_kernel<3,5,256> (4 launches per step)",
+                     "kernel": "vt::stage_kernel<3,5,%d> (4 launches per step)" % block,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3,
                      "stage_ms_single_launches": [float(x) for x in stage_ms]},

@@ -133,8 +133,8 @@ int vt_stage_kernel_times(vt_ctx *ctx, double dt, int nsteps, double *ms_per_sta
  * the reference's error number. */
 int vt_numerical_flux(const double *records, double *flux /* [n][4] */, int *codes, int n);
 
-/* Launch shape of the stage kernels: block = elements per workgroup (128 or
- * 256); persistent != 0 runs 8 * slots_per_xcd resident workgroups that loop
+/* Launch shape of the stage kernels: block = elements per workgroup (64, the
+ * default, 128 or 256); persistent != 0 runs 8 * slots_per_xcd resident workgroups that loop
  * over the element tiles instead of one workgroup per tile.  Results do not
  * depend on it.  No reference counterpart (the reference's OpenMP loop,
  * Evolve.cpp:7-9, has no launch shape). */

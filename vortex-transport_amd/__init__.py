@@ -393,7 +393,7 @@ class Solver:
         _check(lib().vt_stage_kernel_times(self._h, dt, nsteps, _d(ms)), self)
         return ms
 
-    def set_launch(self, block=256, persistent=False, slots_per_xcd=0):
+    def set_launch(self, block=64, persistent=False, slots_per_xcd=0):
         _check(lib().vt_set_launch(self._h, block, int(bool(persistent)), slots_per_xcd))
 
     # --- discrete adjoint (no reference counterpart) ---------------------
@@ -536,7 +536,7 @@ class Batch:
         _check(lib().vt_batch_step_timed(self._h, method, dt, nsteps, ctypes.byref(ms)), self)
         return ms.value
 
-    def set_launch(self, block=256, persistent=False, slots_per_xcd=0):
+    def set_launch(self, block=64, persistent=False, slots_per_xcd=0):
         _check(lib().vt_batch_set_launch(self._h, block, int(bool(persistent)), slots_per_xcd))
 
     def launch(self):

@@ -484,7 +484,7 @@ int vt_stage_kernel_times(vt_ctx* c, double dt, int nsteps, double* ms) {
 }
 
 int vt_set_launch(vt_ctx* c, int block, int persistent, int slots_per_xcd) {
-    if (!c || (block != 128 && block != 256) || slots_per_xcd < 0 || (persistent && slots_per_xcd < 1))
+    if (!c || (block != 64 && block != 128 && block != 256) || slots_per_xcd < 0 || (persistent && slots_per_xcd < 1))
         return VT_ERR_ARG;
     c->cfg.block = block;
     c->cfg.persistent = persistent ? 1 : 0;

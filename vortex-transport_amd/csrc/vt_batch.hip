@@ -403,7 +403,7 @@ int vt_batch_last_error(const vt_batch* b, int* info) {
 }
 
 int vt_batch_set_launch(vt_batch* b, int block, int persistent, int slots_per_xcd) {
-    if (!b || (block != 128 && block != 256) || slots_per_xcd < 0 || (persistent && slots_per_xcd < 1))
+    if (!b || (block != 64 && block != 128 && block != 256) || slots_per_xcd < 0 || (persistent && slots_per_xcd < 1))
         return VT_ERR_ARG;
     b->cfg.block = block;
     b->cfg.persistent = persistent ? 1 : 0;
@@ -426,9 +426,10 @@ int vt_batch_get_launch(const vt_batch* b, int* cfg) {
 int vt_batch_autotune(vt_batch* b, double dt, int trial_steps, int* best_cfg, double* best_ms) {
     if (!b || trial_steps < 1 || trial_steps > vti::kGraphSteps) return VT_ERR_ARG;
     struct Cand { int block, persistent, slots; };
-    // Workgroups resident per CU at 256 threads: 2 (LDS 80 KB and ~250 VGPRs
-    // each at p = 3); 32 CUs per XCD.
-    const Cand cands[] = {{256, 0, 0}, {128, 0, 0}, {256, 1, 64}, {256, 1, 32}, {128, 1, 128}, {128, 1, 64}};
+    // Workgroups resident per CU at p = 3 (LDS 320 B and ~250 VGPRs per
+    // element): 2 of 256 threads, 4 of 128, 8 of 64; 32 CUs per XCD.
+    const Cand cands[] = {{64, 0, 0},   {128, 0, 0},  {256, 0, 0},  {64, 1, 256},
+                          {128, 1, 128}, {256, 1, 64}, {128, 1, 64}, {256, 1, 32}};
     std::vector<double*> saved(b->g.size(), nullptr);
     auto cleanup = [&]() { for (double* s : saved) if (s) hipFree(s); };
     for (size_t i = 0; i < b->g.size(); ++i) {

@@ -5,8 +5,9 @@
 
 namespace vt {
 
-// Default elements (threads) per workgroup; 128 is the other supported tile.
-constexpr int kBlock = 256;
+// Default elements (threads) per workgroup: one wavefront.  128 and 256 are
+// the other supported tiles (more in-block neighbours, fewer workgroups).
+constexpr int kBlock = 64;
 constexpr int kMaxGroups = 4;
 
 enum StageMode : int {

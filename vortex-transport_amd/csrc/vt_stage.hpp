@@ -481,6 +481,7 @@ hipError_t launch_bs(const StageParams& prm, int p, const LaunchConfig& cfg, hip
 
 template <int NQ>
 hipError_t NqLaunch<NQ>::stage(const StageParams& prm, int p, const LaunchConfig& cfg, hipStream_t st) {
+    if (cfg.block == 64) return launch_bs<NQ, 64>(prm, p, cfg, st);
     return cfg.block == 128 ? launch_bs<NQ, 128>(prm, p, cfg, st) : launch_bs<NQ, 256>(prm, p, cfg, st);
 }
 
@@ -489,7 +490,9 @@ hipError_t NqLaunch<NQ>::batched(const BatchParams& bp, const LaunchConfig& cfg,
     const int per_xcd = bp.kbegin[bp.ngroups];
     const int slots = cfg.persistent && cfg.slots_per_xcd > 0 && cfg.slots_per_xcd < per_xcd ? cfg.slots_per_xcd : per_xcd;
     if (slots < 1) return hipSuccess;
-    if (cfg.block == 128)
+    if (cfg.block == 64)
+        hipLaunchKernelGGL((batched_stage_kernel<NQ, 64>), dim3(8 * slots), dim3(64), 0, st, bp);
+    else if (cfg.block == 128)
         hipLaunchKernelGGL((batched_stage_kernel<NQ, 128>), dim3(8 * slots), dim3(128), 0, st, bp);
     else
         hipLaunchKernelGGL((batched_stage_kernel<NQ, 256>), dim3(8 * slots), dim3(256), 0, st, bp);
