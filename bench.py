@@ -42,6 +42,11 @@ HBM_PEAK_GBS = 8000.0
 # i.e. 1280 B per element per stage launch on average.
 BYTES_PER_ELEM_STEP = 5120.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_stage_kernel_p3_512x512.txt")
+FP64_FILE = os.path.join(ROOT, "profiles", "pmc_fp64_stage_kernel_p3_512x512.txt")
+FP64_PEAK_TFLOPS = 78.6
+# Algorithmic FP64 operations per element per stage at p = 3 (traces, 15 Roe
+# fluxes, lift, volume, nodal flux, residual, M^-1, update; FMA = 2).
+FLOPS_PER_ELEM_STAGE = 7565.0
 
 
 def parse_args():
@@ -81,19 +86,24 @@ def profile_pass(s, dt):
                       "copy_bytes_written": b.numel() * 8, "elements": s.ne}))
 
 
-def traffic_from_profile(n_elem):
-    """HBM bytes per stage launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), scaled by the copy-kernel calibration, read from the
-    committed summary; None when it is absent or for another mesh size."""
+def profile_value(path, key, n_elem):
+    """A per-launch figure from a committed rocprofv3 --pmc summary, or None
+    when the summary is absent or for another mesh size."""
     try:
-        txt = open(TRAFFIC_FILE).read()
+        txt = open(path).read()
     except OSError:
         return None
-    m = re.search(r"^bytes_per_launch_avg:\s*([0-9.eE+]+)", txt, re.M)
+    m = re.search(r"^%s:\s*([0-9.eE+]+)" % key, txt, re.M)
     e = re.search(r"^elements:\s*(\d+)", txt, re.M)
     if not m or not e or int(e.group(1)) != n_elem:
         return None
     return float(m.group(1))
+
+
+def traffic_from_profile(n_elem):
+    """HBM bytes per stage launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), scaled by the copy-kernel calibration."""
+    return profile_value(TRAFFIC_FILE, "bytes_per_launch_avg", n_elem)
 
 
 def batched_figure(vt, steps, dt=1e-4):
@@ -314,6 +324,7 @@ def main():
     bytes_per_launch = ne * BYTES_PER_ELEM_STEP / 4
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = traffic_from_profile(ne)
+    fp64_ops = profile_value(FP64_FILE, "fp64_ops_per_launch_avg", ne)
 
     dof_stage = ne * npn * 4 * 4
     line = {
@@ -339,6 +350,11 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3,
                      "stage_ms_single_launches": [float(x) for x in stage_ms]},
+        "fp64": {"algorithmic": FLOPS_PER_ELEM_STAGE * ne / avg_launch_s / 1e12,
+                 "executed": None if fp64_ops is None else fp64_ops / avg_launch_s / 1e12,
+                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": FLOPS_PER_ELEM_STAGE * ne / avg_launch_s / 1e12 / FP64_PEAK_TFLOPS,
+                 "executed_ops_per_launch": fp64_ops},
     }
     s.close()
     if not a.no_batched:
