@@ -249,34 +249,54 @@ def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
             "setup_s": {"mesh": c1 - c0, "create": c2 - c1, "initial_state": c3 - c2}}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(mesh, p, order, steps):
-    """The reference's own translation units (oracle/_ref/vt_ref_run, OpenMP)
-    timed on this host over `steps` RK4 steps of the same mesh; falls back to
-    the oracle port when the reference build is absent."""
+    """The reference's own translation units (oracle/_ref/vt_ref_run, OpenMP,
+    OMP_PROC_BIND=close, OMP_PLACES=cores, OMP_SCHEDULE=static) timed on this
+    host over `steps` RK4 steps of the same mesh on all threads, and over one
+    step on one thread; falls back to the oracle port when the reference build
+    is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     ne = mesh.num_elements
     npn = (p + 1) * (p + 2) // 2
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    sample = "%d RK4 steps of the same %d-element mesh" % (steps, ne)
-    if os.path.exists(O.REF_RUN):
+    omp = {"OMP_PROC_BIND": "close", "OMP_PLACES": "cores", "OMP_SCHEDULE": "static"}
+    host = {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    dof = ne * npn * 4 * 4
+
+    def ref_seconds(nsteps, nthreads):
         with tempfile.TemporaryDirectory() as d:
             r = O.run_reference(d, dict(p=p, nx=1, ny=1, integration_order=order, simulation_time="0.2",
-                                        number_time_steps=2000), steps,
+                                        number_time_steps=2000), nsteps,
                                 mesh=(mesh.element_coordinates, mesh.elements_at_boundary, mesh.element_type),
-                                threads=threads, no_state=True)
-        sec = float(re.search(r"stepping_sec=([0-9.]+)", r["stdout"]).group(1))
-        kind = "reference"
-    else:
-        s = O.Solver(mesh.element_coordinates, mesh.elements_at_boundary, mesh.element_type, p, order)
-        s.init_vortex(0)
-        t0 = time.perf_counter()
-        s.run(1, 1e-4, steps)
-        sec = time.perf_counter() - t0
-        kind = "port"
-    return {"value": ne * npn * 4 * 4 * steps / sec, "unit": "DOF-stage updates/s", "cores": threads,
-            "kind": kind, "sample": sample, "seconds": round(sec, 3)}
+                                threads=nthreads, no_state=True, env_extra=omp)
+        return float(re.search(r"stepping_sec=([0-9.]+)", r["stdout"]).group(1))
+
+    if os.path.exists(O.REF_RUN):
+        sec = ref_seconds(steps, threads)
+        sec1 = ref_seconds(1, 1)
+        return dict({"value": dof * steps / sec, "unit": "DOF-stage updates/s", "cores": threads, "kind": "reference",
+                     "sample": "%d RK4 steps of the same %d-element mesh" % (steps, ne), "seconds": round(sec, 3),
+                     "single_thread": {"value": dof / sec1, "sample": "1 RK4 step, 1 thread", "seconds": round(sec1, 3)},
+                     "omp": omp}, **host)
+    s = O.Solver(mesh.element_coordinates, mesh.elements_at_boundary, mesh.element_type, p, order)
+    s.init_vortex(0)
+    t0 = time.perf_counter()
+    s.run(1, 1e-4, steps)
+    sec = time.perf_counter() - t0
+    return dict({"value": dof * steps / sec, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
+                 "sample": "%d RK4 steps of the same %d-element mesh" % (steps, ne), "seconds": round(sec, 3)}, **host)
 
 
 def main():

@@ -311,7 +311,8 @@ def write_input(path, p=3, nx=8, ny=8, domain_x="10.0", domain_y="10.0", perturb
                 (write_every_steps, U_initialization_type, stepping_method))
 
 
-def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, threads=None, no_state=False):
+def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, threads=None, no_state=False,
+                  env_extra=None):
     """Runs oracle/_ref/vt_ref_run (the reference's own TUs) in workdir; returns a dict of arrays."""
     if not os.path.exists(REF_RUN):
         raise FileNotFoundError(REF_RUN)
@@ -344,6 +345,8 @@ def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, thr
     env = dict(os.environ)
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
+    if env_extra:
+        env.update(env_extra)
     r = subprocess.run(cmd, cwd=workdir, capture_output=True, text=True, env=env)
     if r.returncode != 0:
         raise RuntimeError("reference run failed: %s %s" % (r.stdout[-2000:], r.stderr[-2000:]))
