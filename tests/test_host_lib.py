@@ -164,3 +164,21 @@ def test_batch_adjoint_entry_points_reject_null_handles(vt):
     assert L.vt_batch_mass_l2_objective(None, vt._d(d), vt._d(d), vt._d(d)) == vt.VT_ERR_ARG
     assert L.vt_batch_vortex_gradient(None, 0.0, vt._d(d), vt._d(d), vt._d(d), vt._d(d), vt._d(d)) == vt.VT_ERR_ARG
     assert L.vt_batch_exact_vortex(None, 0, 0.0, 0.3, 0.0, 0.0, vt._d(d)) == vt.VT_ERR_ARG
+
+
+def test_sizes_beyond_the_offset_limit_are_refused_before_any_array_is_read(vt):
+    # 32-bit byte offsets within a plane set: n_e * 4 * Np * 8 B < 4 GiB, i.e.
+    # n_e below ~13.4 M at p = 3.  The limit is checked before the arrays are
+    # touched, so one-element stand-ins suffice (nothing reaches the GPU).
+    L = vt.lib()
+    v = np.zeros(6)
+    b = np.zeros(3, dtype=np.int32)
+    t = np.zeros(1, dtype=np.int32)
+    h = vt.ctypes.c_void_p()
+    assert L.vt_create(14_000_000, vt._d(v), vt._i(b), vt._i(t), 3, 8, vt.ctypes.byref(h)) == vt.VT_ERR_ARG
+    assert not h.value
+    n = np.array([1_500_000_000, 1_000_000_000], dtype=np.int32)
+    pp = np.array([3, 3], dtype=np.int32)
+    oo = np.array([8, 8], dtype=np.int32)
+    assert L.vt_batch_create(2, vt._i(n), vt._i(pp), vt._i(oo), vt._d(v), vt._i(b), vt._i(t),
+                             vt.ctypes.byref(h)) == vt.VT_ERR_ARG

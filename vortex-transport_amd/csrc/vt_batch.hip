@@ -250,6 +250,12 @@ int vt_batch_create(int n_inst, const int* n_elem, const int* p, const int* orde
         if (p[i] < 0 || p[i] > vt::kMaxP) return VT_ERR_UNSUPPORTED_P;
         if (vt::line_rule(order[i], nullptr) < 0) return VT_ERR_QUAD;
         first[i + 1] = first[i] + n_elem[i];
+    }
+    // Whole-ensemble size limit (int element indices, 40-bit error key) before
+    // any array is read; each group's plane-offset limit is checked when it is
+    // created.
+    if (first[n_inst] >= (1ll << 31) || first[n_inst] * 256 >= (1ll << 40)) return VT_ERR_ARG;
+    for (int i = 0; i < n_inst; ++i) {
         for (long long e = first[i]; e < first[i + 1]; ++e)
             for (int s = 0; s < 3; ++s)
                 if (neighbors[3 * e + s] < 0 || neighbors[3 * e + s] >= n_elem[i]) return VT_ERR_ARG;
