@@ -19,7 +19,7 @@ the forward solve alone), "batched_adjoint" (32 uneven instances up to
 gradients, against the batched forward) and "large_mesh" (a forward-only
 1024 x 1024-cell run).
 
-Prints one JSON line (rank 0).  --gpus N > 1 exits: this build runs on one GPU.
+Prints one JSON line (rank 0).  --gpus N > 1 exits: the implementation runs on one GPU.
 """
 import argparse
 import json
@@ -302,7 +302,7 @@ def cpu_baseline(mesh, p, order, steps):
 def main():
     a = parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported" % a.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
     import __graft_entry__ as ge
