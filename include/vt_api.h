@@ -133,6 +133,14 @@ int vt_stage_kernel_times(vt_ctx *ctx, double dt, int nsteps, double *ms_per_sta
  * the reference's error number. */
 int vt_numerical_flux(const double *records, double *flux /* [n][4] */, int *codes, int n);
 
+/* L2 state-error metric of plottools/l2stateerror.py:31-99 between the current
+ * state and U_ref [n][Np][4] (the script's first dump): err[k] =
+ * sqrt(I_k / 25), I_k = sum_e detJ_e sum_i w_i sum_j phi_j(x_i) (U - U_ref)_jk^2
+ * over the 25 points of the order-10 area rule.  The script hard-codes the
+ * p = 3 basis; here phi is the context's basis (the same at p = 3).  Computed
+ * on the device from full-precision values (the script reads 6-decimal text). */
+int vt_l2_state_error(vt_ctx *ctx, const double *U_ref, double err[4]);
+
 /* Launch shape of the stage kernels: block = elements per workgroup (64, the
  * default, 128 or 256); persistent != 0 runs 8 * slots_per_xcd resident workgroups that loop
  * over the element tiles instead of one workgroup per tile.  Results do not

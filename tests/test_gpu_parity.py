@@ -131,6 +131,10 @@ def test_full_period_l2_error_matches_reference(vt, oracle, gpu):
     err = oracle.l2_state_error(d["vertices"], d["U0"], U)
     ref = d["l2_state_error"]
     assert np.all(np.abs(err - ref) / ref < 5e-4), (err, ref)  # 3 significant digits
+    # the library's own device-side metric: the oracle's to summation order
+    dev = s.l2_state_error(d["U0"])
+    assert np.all(np.abs(dev - err) <= 1e-12 * err), (dev, err)
+    assert np.all(np.abs(dev - ref) / ref < 5e-4)
     txt = np.vectorize(lambda x: float("%f" % x))
     err_t = oracle.l2_state_error(txt(d["vertices"]), txt(d["U0"]), txt(U))
     assert np.all(np.abs(err_t - d["l2_state_error_text"]) / d["l2_state_error_text"] < 5e-4)
@@ -293,3 +297,33 @@ def test_every_quadrature_size_matches_oracle(vt, oracle, gpu, p, order):
         s.rk4(2e-3, 6)
         assert rel(s.state(), o.state()) <= 1e-12
     s.close()
+
+
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+def test_l2_state_error_matches_oracle(vt, oracle, gpu, p):
+    # ragged block count (600 elements), every polynomial order (the script's
+    # p = 3 basis generalised to the context's order), perturbed grid
+    mesh = vt.generate_mesh(nx=20, ny=15, perturb_grid=1.0)
+    s = vt.Solver(mesh, p, 8)
+    s.initialize_hydrodinamics(0)
+    U0 = s.state()
+    s.rk4(1e-3, 30)
+    err = s.l2_state_error(U0)
+    assert np.all(err > 0)
+    assert np.array_equal(s.l2_state_error(s.state()), np.zeros(4))
+    # closed form: a difference c_k constant over the mesh interpolates to c_k^2
+    # at every point (partition of unity), the rule's weights sum to 1/2 and the
+    # detJ sum to twice the mesh area A, so err_k = |c_k| sqrt(A / 25).  The
+    # shifts are powers of two that keep each variable inside its binade
+    # (rho in [0.99, 1], u, v in [0.5, 1), rho E in [4, 8)), so U + c - U = c
+    # exactly.
+    c = np.array([-2.0 ** -10, 2.0 ** -11, -2.0 ** -11, 3 * 2.0 ** -10])
+    UT = s.state()
+    s.set_state(UT + c)
+    v = mesh.element_coordinates.reshape(-1, 6)
+    area = 0.5 * np.sum((v[:, 2] - v[:, 0]) * (v[:, 5] - v[:, 1]) - (v[:, 3] - v[:, 1]) * (v[:, 4] - v[:, 0]))
+    assert np.allclose(s.l2_state_error(UT), np.abs(c) * np.sqrt(area / 25.0), rtol=1e-12, atol=0)
+    s.set_state(UT)
+    if p == 3:
+        ref = oracle.l2_state_error(mesh.element_coordinates, U0, s.state())
+        assert np.all(np.abs(err - ref) <= 1e-12 * ref), (err, ref)

@@ -103,6 +103,7 @@ def lib():
             "vt_mass_l2_objective": (ctypes.c_int, [vp, _dp, _dp, _dp]),
             "vt_adjoint": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
             "vt_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "vt_l2_state_error": (ctypes.c_int, [vp, _dp, _dp]),
             "vt_get_launch": (ctypes.c_int, [vp, _ip]),
             "vt_batch_create": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip, _dp, _ip, _ip, ctypes.POINTER(vp)]),
             "vt_batch_destroy": (None, [vp]),
@@ -421,6 +422,14 @@ class Solver:
         out = np.zeros((3, self.ne, self.np, 4))
         _check(lib().vt_vortex_tangent(self._h, t, epsilon, x0, y0, _d(out)))
         return out
+
+    def l2_state_error(self, U_ref):
+        """The L2 state-error metric of plottools/l2stateerror.py between the
+        current state and U_ref [n][Np][4] (four components), on the device."""
+        U = np.ascontiguousarray(U_ref, dtype=np.float64).reshape(self.ne, self.np, 4)
+        err = np.zeros(4)
+        _check(lib().vt_l2_state_error(self._h, _d(U), _d(err)))
+        return err
 
     def mass_l2_objective(self, rho_target):
         """(J, dJ/dU) of J = sum_e detJ_e drho^T M_ref drho at the current state."""

@@ -2,6 +2,7 @@
 // RK4 / forward-Euler stepping through captured HIP graphs, error reporting.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -352,7 +353,8 @@ void vt_destroy(vt_ctx* c) {
     for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
     for (void* b : {static_cast<void*>(c->Un), static_cast<void*>(c->Ua), static_cast<void*>(c->Ub),
                     static_cast<void*>(c->acc), static_cast<void*>(c->vtx), static_cast<void*>(c->nbr),
-                    static_cast<void*>(c->tab), static_cast<void*>(c->err)})
+                    static_cast<void*>(c->tab), static_cast<void*>(c->err), static_cast<void*>(c->l2ref),
+                    static_cast<void*>(c->l2tab), static_cast<void*>(c->l2part)})
         if (b) hipFree(b);
     if (c->err_init) hipHostFree(c->err_init);
     if (c->err_host) hipHostFree(c->err_host);
@@ -387,6 +389,46 @@ int vt_set_state(vt_ctx* c, const double* U) {
 int vt_get_state(vt_ctx* c, double* U) {
     if (!c || !U) return VT_ERR_ARG;
     return read_planes(c, c->Un, U);
+}
+
+int vt_l2_state_error(vt_ctx* c, const double* U_ref, double* err) {
+    if (!c || !U_ref || !err) return VT_ERR_ARG;
+    const int nblk = (c->ne + vt::kL2Block - 1) / vt::kL2Block;
+    if (!c->l2ref) {
+        std::vector<double> q;
+        if (vt::area_rule(10, &q) != vt::kL2Points) return VT_ERR_QUAD;
+        std::vector<double> t(static_cast<size_t>(vt::kL2Points) * (c->np + 1));
+        for (int i = 0; i < vt::kL2Points; ++i) {
+            vt::basis(c->p, q[3 * i], q[3 * i + 1], &t[static_cast<size_t>(i) * c->np]);
+            t[static_cast<size_t>(vt::kL2Points) * c->np + i] = q[3 * i + 2];
+        }
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->l2ref), sizeof(double) * 4 * c->np * c->ld));
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->l2tab), sizeof(double) * t.size()));
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->l2part), sizeof(double) * 4 * nblk));
+        HIP_TRY(hipMemcpyAsync(c->l2tab, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    int rc = vti::upload_state(c, U_ref, c->l2ref);
+    if (rc) return rc;
+    vt::L2Params l{};
+    l.U = c->Un;
+    l.Uref = c->l2ref;
+    l.vtx = c->vtx;
+    l.tab = c->l2tab;
+    l.part = c->l2part;
+    l.ne = c->ne;
+    l.ld = c->ld;
+    HIP_TRY(vt::launch_l2(l, c->p, c->stream));
+    double* h = vti::host_stage(c, 4 * static_cast<size_t>(nblk));
+    if (!h) return VT_ERR_HIP;
+    HIP_TRY(hipMemcpyAsync(h, c->l2part, sizeof(double) * 4 * nblk, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; ++k) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += h[4 * b + k];
+        err[k] = std::sqrt((1.0 / 25.0) * s);
+    }
+    return VT_OK;
 }
 
 int vt_get_flux(vt_ctx* c, double* F) {

@@ -20,6 +20,9 @@ struct vt_ctx {
     unsigned long long* err_init = nullptr;  // pinned host, holds kNoError
     unsigned long long* err_host = nullptr;  // pinned host, one slot per replay
     size_t err_host_cap = 0;
+    double* l2ref = nullptr;                 // reference state planes of the L2 metric
+    double* l2tab = nullptr;                 // basis at the metric's 25 points, weights
+    double* l2part = nullptr;                // metric partial sums, [blocks][4]
     double* host_stage = nullptr;            // pinned host staging for plane transfers
     size_t host_stage_cap = 0;               // in doubles
     std::vector<double> vertices;

@@ -95,11 +95,28 @@ struct ObjParams {
     int ne, ld;
 };
 
+// L2 state-error metric (plottools/l2stateerror.py:31-99): per element the
+// squared nodal difference of two states is interpolated to the 25 points of
+// the order-10 area rule, weighted and scaled by detJ; each block of kL2Block
+// elements writes its four component sums to part[4 * block + k].
+constexpr int kL2Points = 25;
+constexpr int kL2Block = 256;
+
+struct L2Params {
+    const double* U;     // state planes
+    const double* Uref;  // reference state planes
+    const double* vtx;   // vertex planes
+    const double* tab;   // basis values [kL2Points][Np], then the rule's weights [kL2Points]
+    double* part;        // [blocks][4]
+    int ne, ld;
+};
+
 hipError_t launch_stage(const StageParams& prm, int p, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st);
 hipError_t launch_batched(const BatchParams& bp, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st);
 hipError_t launch_objective(const ObjParams& prm, int p, hipStream_t st);
+hipError_t launch_l2(const L2Params& prm, int p, hipStream_t st);
 
 inline int tiles_per_xcd(int ne, int block) { return ((ne + block - 1) / block + 7) / 8; }
 inline int blocks_per_xcd(int ne) { return tiles_per_xcd(ne, kBlock); }

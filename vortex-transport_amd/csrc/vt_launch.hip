@@ -57,6 +57,55 @@ __global__ void __launch_bounds__(kObjBlock) objective_kernel(ObjParams prm) {
     if (threadIdx.x == 0) prm.part[blockIdx.x] = red[0];
 }
 
+// The L2 state-error metric (see L2Params).  Per element and component:
+// sum over the rule's points of w_i * sum_j phi_j(x_i) (U - Uref)_j^2, in that
+// order, times detJ; the block sum is a fixed-shape tree.
+template <int P>
+__global__ void __launch_bounds__(kL2Block) l2_kernel(L2Params prm) {
+    constexpr int NP = (P + 1) * (P + 2) / 2;
+    __shared__ double red[4 * kL2Block];
+    const int e = blockIdx.x * kL2Block + static_cast<int>(threadIdx.x);
+    double ik[4] = {0.0, 0.0, 0.0, 0.0};
+    if (e < prm.ne) {
+        const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
+        const unsigned oe = static_cast<unsigned>(e) * 8u;
+        const ctab_t B = const_table(prm.tab);
+        const double x0 = plane_at(prm.vtx, 0, ld8, oe), y0 = plane_at(prm.vtx, 1, ld8, oe);
+        const double x1 = plane_at(prm.vtx, 2, ld8, oe), y1 = plane_at(prm.vtx, 3, ld8, oe);
+        const double x2 = plane_at(prm.vtx, 4, ld8, oe), y2 = plane_at(prm.vtx, 5, ld8, oe);
+        const double det = (x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            double du[NP];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const double d = plane_at(prm.U, k * NP + j, ld8, oe) - plane_at(prm.Uref, k * NP + j, ld8, oe);
+                du[j] = d * d;
+            }
+            double sum = 0.0;
+#pragma unroll 5
+            for (int i = 0; i < kL2Points; ++i) {
+                double dq = 0.0;
+#pragma unroll
+                for (int j = 0; j < NP; ++j) dq += du[j] * B[i * NP + j];
+                sum += B[kL2Points * NP + i] * dq;
+            }
+            ik[k] = det * sum;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[k * kL2Block + threadIdx.x] = ik[k];
+    __syncthreads();
+#pragma unroll
+    for (int w = kL2Block / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) red[k * kL2Block + threadIdx.x] += red[k * kL2Block + threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) prm.part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x * kL2Block];
+}
+
 #define VT_NQ_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
 
 // The launchers (and their kernels) are instantiated in vt_kernels_nq.hip,
@@ -95,6 +144,18 @@ hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st) {
 hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(flux_kernel, dim3((n + 255) / 256), dim3(256), 0, st, in, out, codes, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_l2(const L2Params& prm, int p, hipStream_t st) {
+    const dim3 grid((prm.ne + kL2Block - 1) / kL2Block), block(kL2Block);
+    switch (p) {
+    case 0: hipLaunchKernelGGL(l2_kernel<0>, grid, block, 0, st, prm); break;
+    case 1: hipLaunchKernelGGL(l2_kernel<1>, grid, block, 0, st, prm); break;
+    case 2: hipLaunchKernelGGL(l2_kernel<2>, grid, block, 0, st, prm); break;
+    case 3: hipLaunchKernelGGL(l2_kernel<3>, grid, block, 0, st, prm); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
