@@ -207,6 +207,7 @@ def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
     b.time = 0.0
     t0 = time.perf_counter()
     J, lam = b.adjoint(dt, steps, tg, every=every)
+    t1 = time.perf_counter()
     grad = b.vortex_gradient(0.0, eps, x0, y0, lam)
     ta = time.perf_counter() - t0
     ne = sum(b.ne)
@@ -217,6 +218,7 @@ def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
                         "integration_order 8, %d RK4 steps of dt %g, checkpoints every %d steps" % (steps, dt, every),
             "elements": ne, "steps": steps, "forward_s": tf, "forward_ms_per_step": tf / steps * 1e3,
             "forward_plus_adjoint_s": ta, "ratio": ta / tf, "launches_per_stage": 1,
+            "parameter_gradient_s": ta - (t1 - t0),
             "device_state_gb": slots * slot_bytes / 1e9, "J_sum": float(np.sum(J)),
             "grad_finite": bool(np.isfinite(grad).all()), "grad_abs_max": float(np.abs(grad).max())}
 

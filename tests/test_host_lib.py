@@ -182,3 +182,12 @@ def test_sizes_beyond_the_offset_limit_are_refused_before_any_array_is_read(vt):
     oo = np.array([8, 8], dtype=np.int32)
     assert L.vt_batch_create(2, vt._i(n), vt._i(pp), vt._i(oo), vt._d(v), vt._i(b), vt._i(t),
                              vt.ctypes.byref(h)) == vt.VT_ERR_ARG
+
+
+def test_views_of_one_buffer_are_passed_back_without_a_copy(vt):
+    flat = np.arange(4 * (3 * 10 + 2 * 6), dtype=np.float64)
+    parts = [flat[0:120].reshape(3, 10, 4), flat[120:168].reshape(2, 6, 4)]
+    assert vt._joined(parts, flat.size) is flat
+    assert vt._joined([p.copy() for p in parts], flat.size) is None   # not views
+    assert vt._joined(parts[::-1], flat.size) is None                 # out of order
+    assert vt._joined(parts[:1], flat.size) is None                   # incomplete

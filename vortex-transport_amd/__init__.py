@@ -589,7 +589,7 @@ class Batch:
         [n_i][Np_i] density per instance."""
         rt = self._targets(rho_targets)
         J = np.zeros(self.n)
-        g = np.zeros(4 * rt.size)
+        g = np.empty(4 * rt.size)  # every entry is written
         _check(lib().vt_batch_mass_l2_objective(self._h, _d(rt), _d(J), _d(g)), self)
         return J, self._split(g)
 
@@ -598,7 +598,7 @@ class Batch:
         the reverse sweep: (J [n], [dJ_i/dU0_i])."""
         rt = self._targets(rho_targets)
         J = np.zeros(self.n)
-        g = np.zeros(4 * rt.size)
+        g = np.empty(4 * rt.size)  # every entry is written
         _check(lib().vt_batch_adjoint(self._h, dt, nsteps, every, _d(rt), _d(J), _d(g)), self)
         return J, self._split(g)
 
@@ -608,12 +608,32 @@ class Batch:
         e = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (self.n,)))
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(x0, dtype=np.float64), (self.n,)))
         c = np.ascontiguousarray(np.broadcast_to(np.asarray(y0, dtype=np.float64), (self.n,)))
-        lam = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in dJdU0]))
-        if lam.size != 4 * sum(self.ne[i] * self.np[i] for i in range(self.n)):
+        total = 4 * sum(self.ne[i] * self.np[i] for i in range(self.n))
+        lam = _joined(dJdU0, total)
+        if lam is None:
+            lam = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in dJdU0]))
+        if lam.size != total:
             raise ValueError("dJdU0 needs one [n_i][Np_i][4] array per instance")
         grad = np.zeros((self.n, 3))
         _check(lib().vt_batch_vortex_gradient(self._h, t, _d(e), _d(a), _d(c), _d(lam), _d(grad)))
         return grad
+
+
+def _joined(arrays, total):
+    """The flat array that a list of consecutive views was split from (as
+    Batch.adjoint returns them), or None: lets a caller pass the list back
+    without a copy."""
+    base = getattr(arrays[0], "base", None) if len(arrays) else None
+    if not isinstance(base, np.ndarray) or base.dtype != np.float64 or base.ndim != 1 or base.size != total:
+        return None
+    if not base.flags.c_contiguous:
+        return None
+    addr = base.__array_interface__["data"][0]
+    for x in arrays:
+        if not isinstance(x, np.ndarray) or x.base is not base or x.__array_interface__["data"][0] != addr:
+            return None
+        addr += x.nbytes
+    return base if addr == base.__array_interface__["data"][0] + base.nbytes else None
 
 
 def splitmix64(seed):

@@ -38,7 +38,7 @@ int vti::ensure_adj_planes(vt_ctx* c) {
     return VT_OK;
 }
 
-int vti::objective(vt_ctx* c, const double* rho_target, int nr, const int* e0, const int* n, double* J) {
+int vti::objective(vt_ctx* c, int nr, const int* e0, const int* n, const double* const* rho, double* J) {
     if (c->order < 0) return VT_ERR_QUAD;  // M_ref comes from the integration order's area rule
     int rc = ensure_adj_planes(c);
     if (rc) return rc;
@@ -61,7 +61,7 @@ int vti::objective(vt_ctx* c, const double* rho_target, int nr, const int* e0, c
     }
     double* h = host_stage(c, std::max(static_cast<size_t>(c->np) * c->ld, static_cast<size_t>(boff[nr])));
     if (!h) return VT_ERR_HIP;
-    vt::density_to_planes(c->ne, c->np, c->ld, rho_target, h);
+    for (int r = 0; r < nr; ++r) vt::density_to_planes(n[r], c->np, c->ld, rho[r], h + e0[r]);
     HIP_TRY(hipMemcpyAsync(c->tgt, h, sizeof(double) * c->np * c->ld, hipMemcpyHostToDevice, c->stream));
     // One launch per element range: the plane base pointers move to the
     // range's first element, the plane stride stays ld.
@@ -262,7 +262,7 @@ int vt_vortex_tangent(const vt_ctx* c, double t, double epsilon, double x0, doub
 int vt_mass_l2_objective(vt_ctx* c, const double* rho_target, double* J, double* dJdU) {
     if (!c || !rho_target || !J) return VT_ERR_ARG;
     const int e0 = 0;
-    int rc = vti::objective(c, rho_target, 1, &e0, &c->ne, J);
+    int rc = vti::objective(c, 1, &e0, &c->ne, &rho_target, J);
     if (rc || !dJdU) return rc;
     return download_planes(c, c->adj, dJdU);
 }
@@ -273,7 +273,7 @@ int vt_adjoint(vt_ctx* c, double dt, int nsteps, int every, const double* rho_ta
     if (rc) return rc;
     // The seed dJ/dU_T goes from the objective kernel straight into lam.
     const int e0 = 0;
-    rc = vti::objective(c, rho_target, 1, &e0, &c->ne, J);
+    rc = vti::objective(c, 1, &e0, &c->ne, &rho_target, J);
     if (rc) return rc;
     rc = vti::backward_sweep(c);
     if (rc) return rc;

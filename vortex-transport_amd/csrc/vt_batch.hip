@@ -207,31 +207,31 @@ std::vector<size_t> node_offsets(const vt_batch* b) {
 int group_objective(vt_batch* b, int gi, const double* rho_target, const std::vector<size_t>& off, double* J) {
     vt_ctx* c = b->g[gi];
     const std::vector<int>& mem = b->members[gi];
-    std::vector<double> tg(static_cast<size_t>(c->ne) * c->np);
     std::vector<int> e0(mem.size()), n(mem.size());
+    std::vector<const double*> rho(mem.size());
     for (size_t k = 0; k < mem.size(); ++k) {
         const int i = mem[k];
-        std::memcpy(&tg[static_cast<size_t>(b->eoff[i]) * c->np], rho_target + off[i],
-                    sizeof(double) * static_cast<size_t>(b->ne[i]) * c->np);
         e0[k] = b->eoff[i];
         n[k] = b->ne[i];
+        rho[k] = rho_target + off[i];
     }
     std::vector<double> Jg(mem.size());
-    const int rc = vti::objective(c, tg.data(), static_cast<int>(mem.size()), e0.data(), n.data(), Jg.data());
+    const int rc = vti::objective(c, static_cast<int>(mem.size()), e0.data(), n.data(), rho.data(), Jg.data());
     if (rc) return rc;
     for (size_t k = 0; k < mem.size(); ++k) J[mem[k]] = Jg[k];
     return VT_OK;
 }
 
-// dJ/dU of group gi (its lam planes) into the members' slices of dJdU.
+// dJ/dU of group gi (its lam planes) into the members' slices of dJdU,
+// converted from the planes straight into each slice.
 int scatter_group_gradient(vt_batch* b, int gi, const std::vector<size_t>& off, double* dJdU) {
     vt_ctx* c = b->g[gi];
-    std::vector<double> lg(static_cast<size_t>(c->ne) * c->np * 4);
-    const int rc = vti::download_state(c, c->adj, lg.data());
-    if (rc) return rc;
-    for (int i : b->members[gi])
-        std::memcpy(dJdU + 4 * off[i], &lg[static_cast<size_t>(b->eoff[i]) * c->np * 4],
-                    sizeof(double) * static_cast<size_t>(b->ne[i]) * c->np * 4);
+    const size_t n = static_cast<size_t>(4) * c->np * c->ld;
+    double* h = vti::host_stage(c, n);
+    if (!h) return VT_ERR_HIP;
+    HIP_TRY(hipMemcpyAsync(h, c->adj, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int i : b->members[gi]) vt::planes_to_aos(b->ne[i], c->np, c->ld, b->eoff[i], h, dJdU + 4 * off[i]);
     return VT_OK;
 }
 

@@ -89,10 +89,10 @@ void free_adjoint(vt_ctx* c);
 int ensure_adj_planes(vt_ctx* c);
 
 // Objective of the adjoint over nr element ranges [e0[r], e0[r] + n[r]) of the
-// current state, rho_target [ne][Np] for the whole context: J[r] per range
+// current state, with target densities rho[r] [n[r]][Np]: J[r] per range
 // (block partial sums added in block order) and dJ/dU of every range written
 // into the lam planes.
-int objective(vt_ctx* c, const double* rho_target, int nr, const int* e0, const int* n, double* J);
+int objective(vt_ctx* c, int nr, const int* e0, const int* n, const double* const* rho, double* J);
 
 // The reverse sweep over the recorded forward run, from lam = dJ/dU_T in the
 // lam planes to lam = dJ/dU_0 in the same planes.
