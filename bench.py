@@ -67,6 +67,7 @@ def parse_args():
     ap.add_argument("--no-batched-adjoint", action="store_true")
     ap.add_argument("--batched-adjoint-steps", type=int, default=500)
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-shard", action="store_true")
     ap.add_argument("--large-steps", type=int, default=20)
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 --pmc: 2 steps of per-launch stage kernels, then a plain device copy "
@@ -251,6 +252,28 @@ def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
             "setup_s": {"mesh": c1 - c0, "create": c2 - c1, "initial_state": c3 - c2}}
 
 
+def shard_figure(vt, steps, dt=1e-4, nx=512, ny=64):
+    """One GPU's share of the 512 x 512 workload split into 8 horizontal
+    strips (512 x 64 cells, 65,536 triangles) run as its own periodic mesh:
+    the kernel at the shape each GPU would run; the exchange of the strips'
+    boundary traces is not part of it."""
+    import torch
+    mesh = vt.generate_mesh_seeded(nx, ny, 20.0, 20.0 * ny / nx, 0.02, 2)
+    s = vt.Solver(mesh, 3, 8)
+    s.initialize_hydrodinamics(0)
+    s.rk4(dt, 8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.rk4(dt, steps)
+    wall = time.perf_counter() - t0
+    ne = s.ne
+    s.close()
+    return {"workload": "%dx%d cells (%d triangles): one of 8 strips of the 512x512 mesh, p=3, integration_order 8, "
+                        "RK4, dt %g" % (nx, ny, ne, dt),
+            "elements": ne, "steps": steps, "ms_per_step": wall / steps * 1e3,
+            "value": ne * 10 * 4 * 4 * steps / wall, "unit": "DOF-stage updates/s"}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -387,6 +410,8 @@ def main():
         line["batched_adjoint"] = batched_adjoint_figure(vt, a.batched_adjoint_steps)
     if not a.no_large:
         line["large_mesh"] = large_mesh_figure(vt, a.large_steps)
+    if not a.no_shard:
+        line["shard"] = shard_figure(vt, 200)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mesh, p, order, a.cpu_steps)
     print(json.dumps(line))
