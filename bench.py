@@ -213,7 +213,7 @@ def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
     ta = time.perf_counter() - t0
     ne = sum(b.ne)
     slot_bytes = ne * 10 * 4 * 8
-    slots = -(-steps // every) + 4 * every + 9
+    slots = -(-steps // every) + 4 * every + 8
     b.close()
     return {"workload": "32 instances (seed 5): cells 64..512 per side, p=3, eps 0.1..0.4, 10x10 domain, "
                         "integration_order 8, %d RK4 steps of dt %g, checkpoints every %d steps" % (steps, dt, every),

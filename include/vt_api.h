@@ -224,7 +224,7 @@ int vt_batch_mass_l2_objective(vt_batch *b, const double *rho_target, double *J,
 /* nsteps RK4 steps of dt recorded with a checkpoint every `every` steps
  * (the state advances to U_T as with vt_batch_step), the objectives J [n_inst]
  * at U_T, and the reverse sweep: dJdU0 = dJ_i/dU0_i.  Device memory per
- * (p, order) group of E elements: (ceil(nsteps/every) + 4 every + 9) states
+ * (p, order) group of E elements: (ceil(nsteps/every) + 4 every + 8) states
  * of E * Np * 32 bytes, held only while the group is processed.
  * VT_ERR_NONPHYSICAL as vt_batch_step. */
 int vt_batch_adjoint(vt_batch *b, double dt, int nsteps, int every, const double *rho_target, double *J,

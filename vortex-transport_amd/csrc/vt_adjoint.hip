@@ -6,8 +6,9 @@
 //
 // Memory (one "slot" = one state, 4 * Np * ld doubles):
 //   ckpt      ceil(nsteps / every) slots, the state at the start of each segment;
-//   seg       4 * every + 1 slots, the four stage states of every step of one
-//             segment (recomputed from its checkpoint during the reverse sweep);
+//   seg       4 * every slots, the four stage states of every step of one
+//             segment (recomputed from its checkpoint during the reverse sweep;
+//             the checkpoint itself serves as the first step's start);
 //   adj       4 slots: lam, acc, y (ping-pong pair).
 // The reverse sweep launches, per segment, the forward stage kernels of the
 // segment with their outputs directed into seg (no extra copies), then per step
@@ -105,7 +106,9 @@ int upload_planes(vt_ctx* c, const double* U, double* dev) { return vti::upload_
 
 int download_planes(vt_ctx* c, const double* dev, double* U) { return vti::download_state(c, dev, U); }
 
-// Forward stage kernels of one step from slot S0 into slots S1, S2, S3 and next.
+// Forward stage kernels of one step from slot S0 into slots S1, S2, S3 and
+// next; with next null the fourth stage (whose output the reverse sweep of the
+// step does not read) is skipped.
 hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* S2, double* S3, double* next) {
     vt::StageParams prm{};
     prm.vtx = c->vtx;
@@ -119,7 +122,7 @@ hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* 
     prm.acc = c->acc;
     const double* in[4] = {S0, S1, S2, S3};
     double* out[4] = {S1, S2, S3, next};
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < (next ? 4 : 3); ++s) {
         prm.Uin = in[s];
         prm.Uout = out[s];
         prm.Un = (s == 1 || s == 2) ? S0 : nullptr;
@@ -137,7 +140,7 @@ int vti::backward_sweep(vt_ctx* c) {
     const int every = c->rec_every, nsteps = c->rec_steps, nseg = (nsteps + every - 1) / every;
     const double dt = c->rec_dt;
     const size_t n = slot_elems(c);
-    const int need = 4 * (nsteps < every ? nsteps : every) + 1;
+    const int need = 4 * (nsteps < every ? nsteps : every);
     if (c->seg_slots < need) {
         if (c->seg) (void)hipFree(c->seg);
         c->seg = nullptr;
@@ -152,12 +155,17 @@ int vti::backward_sweep(vt_ctx* c) {
     a.scale = 1.0 / 6.0;
     HIP_TRY(vt::launch_adjoint(a, c->p, c->nq, c->stream));
     int cur = 0;
-    auto slot = [&](int step, int stage) { return c->seg + (static_cast<size_t>(4) * step + stage) * n; };
+    // The segment's first step starts from its checkpoint in place.
+    int seg_i = 0;
+    auto slot = [&](int step, int stage) {
+        return (step == 0 && stage == 0) ? c->ckpt[seg_i] : c->seg + (static_cast<size_t>(4) * step + stage) * n;
+    };
     for (int i = nseg - 1; i >= 0; --i) {
         const int len = (i + 1 < nseg) ? every : nsteps - i * every;
-        HIP_TRY(hipMemcpyAsync(slot(0, 0), c->ckpt[i], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        seg_i = i;
         for (int s = 0; s < len; ++s)
-            HIP_TRY(recompute_step(c, dt, slot(s, 0), slot(s, 1), slot(s, 2), slot(s, 3), slot(s + 1, 0)));
+            HIP_TRY(recompute_step(c, dt, slot(s, 0), slot(s, 1), slot(s, 2), slot(s, 3),
+                                   s + 1 < len ? slot(s + 1, 0) : nullptr));
         for (int s = len - 1; s >= 0; --s)
             for (int st = 3; st >= 0; --st) {
                 vt::AdjParams b = adj_params(c, vt::kAdj1 + st);
