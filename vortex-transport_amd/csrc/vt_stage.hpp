@@ -323,11 +323,14 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         // Two halves of the variables: the second half's loads are issued once
         // the first half's registers are free.
         //   stage 2: U3 = Un + k2/2 with Un = U2 - k1/2 recovered from the staged
-        //            U2 and acc = k1 (exact: Fast2Sum, |k1/2| < |Un|); neither Un
-        //            is read nor acc written.
+        //            U2 and acc = k1; neither Un is read nor acc written.  With
+        //            |k1/2| << |Un| the recovery returns Un itself unless the
+        //            sum U2 rounded to a tie or into the next binade, where it
+        //            can be one ulp off;
         //   stage 3: acc = Un + (k1 + 2 k2 + 2 k3)/6 and U4 = Un + k3, with
-        //            k2 = 2 (U3 - Un) recovered from the staged U3 (U3 - Un is
-        //            exact by Sterbenz; k2 carries U3's rounding).
+        //            k2 = 2 (U3 - Un) recovered from the staged U3 (the
+        //            difference is exact when U3 and Un are within a factor of
+        //            two, Sterbenz; k2 then carries U3's rounding).
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             double ac[2 * NP], u0[2 * NP];
