@@ -7,6 +7,14 @@
  * Every arithmetic expression keeps the operand order of the reference file
  * cited beside it, so that gcc -O2 -ffp-contract=off on x86-64 gives the same
  * bits as the reference translation units built the same way.
+ *
+ * Pinning: the reference's own translation units (oracle/Makefile target ref)
+ * on fixtures for every function of the forward path, plus closed forms and
+ * invariants (tests/test_oracle_pins.py).  One part is parity unpinned at the
+ * last ulp: oro_invert stands where the reference calls Eigen's .inverse()
+ * (Preevolve.cpp:57-65), which is absent here; it is checked against the exact
+ * rational inverse of M_ref (~1e-14), and the fixtures feed both sides the
+ * same inverse.
  */
 #define _DEFAULT_SOURCE
 #include "vt_oracle.h"
