@@ -68,6 +68,7 @@ def parse_args():
     ap.add_argument("--batched-adjoint-steps", type=int, default=500)
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-shard", action="store_true")
+    ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--large-steps", type=int, default=20)
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 --pmc: 2 steps of per-launch stage kernels, then a plain device copy "
@@ -252,6 +253,32 @@ def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
             "setup_s": {"mesh": c1 - c0, "create": c2 - c1, "initial_state": c3 - c2}}
 
 
+def pcie_figure(vt, steps, dt=1e-4, n=512):
+    """The workload stepped as the drop-in's per-step rk4() does it: the host
+    state goes to the device, one RK4 step runs, and U and the nodal flux F come
+    back, every step (PCIe-inclusive; not the resident figure)."""
+    mesh = vt.generate_mesh_seeded(n, n, 20.0, 20.0, 0.02, 2)
+    s = vt.Solver(mesh, 3, 8)
+    s.initialize_hydrodinamics(0)
+    U = s.state()
+    s.rk4(dt, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.set_state(U)
+        s.rk4(dt, 1)
+        U = s.state()
+        F = s.flux()
+    wall = time.perf_counter() - t0
+    ne = s.ne
+    s.close()
+    assert np.isfinite(F).all()
+    return {"workload": "%dx%d cells, as the main figure, one RK4 step per call with the state uploaded and U, F "
+                        "downloaded around it" % (n, n),
+            "steps": steps, "ms_per_step": wall / steps * 1e3,
+            "value": ne * 10 * 4 * 4 * steps / wall, "unit": "DOF-stage updates/s",
+            "bytes_moved_per_step": ne * 10 * 4 * 8 * 2 + ne * 10 * 8 * 8}
+
+
 def shard_figure(vt, steps, dt=1e-4, nx=512, ny=64):
     """One GPU's share of the 512 x 512 workload split into 8 horizontal
     strips (512 x 64 cells, 65,536 triangles) run as its own periodic mesh:
@@ -412,6 +439,8 @@ def main():
         line["large_mesh"] = large_mesh_figure(vt, a.large_steps)
     if not a.no_shard:
         line["shard"] = shard_figure(vt, 200)
+    if not a.no_pcie:
+        line["pcie_inclusive"] = pcie_figure(vt, 5)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mesh, p, order, a.cpu_steps)
     print(json.dumps(line))
