@@ -501,6 +501,10 @@ class Batch:
         """Kernel launches per RK stage."""
         return lib().vt_batch_num_launches(self._h)
 
+    @property
+    def num_instances(self):
+        return lib().vt_batch_num_instances(self._h)
+
     def initialize_hydrodinamics(self, init_type, epsilon, x0, y0):
         e = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (self.n,)))
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(x0, dtype=np.float64), (self.n,)))
