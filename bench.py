@@ -69,6 +69,7 @@ def parse_args():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-shard", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--large-steps", type=int, default=20)
     ap.add_argument("--profile-pass", action="store_true",
                     help="for rocprofv3 --pmc: 2 steps of per-launch stage kernels, then a plain device copy "
@@ -301,6 +302,43 @@ def shard_figure(vt, steps, dt=1e-4, nx=512, ny=64):
             "value": ne * 10 * 4 * 4 * steps / wall, "unit": "DOF-stage updates/s"}
 
 
+def c1_figure(vt, with_cpu):
+    """The reference's own small case run to the end: 8 x 8 cells of the
+    10 x 10 domain (128 triangles), p = 3, integration_order 8, 10,000 RK4
+    steps of dt = 1.414213562373e-3 (T = 10 sqrt 2), time to solution on the
+    GPU and, beside it, the reference's translation units on the host."""
+    import torch
+    prm = {"p": 3, "nx": 8, "ny": 8, "integration_order": 8, "simulation_time": "14.14213562373",
+           "number_time_steps": 10000}
+    dt = float(prm["simulation_time"]) / prm["number_time_steps"]
+    mesh = vt.generate_mesh(nx=8, ny=8)
+    s = vt.Solver(mesh, 3, 8)
+    s.initialize_hydrodinamics(0)
+    s.rk4(dt, 64)
+    s.time = 0.0  # the initial state is set at the solver's time
+    s.initialize_hydrodinamics(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.rk4(dt, prm["number_time_steps"])
+    gpu_s = time.perf_counter() - t0
+    U0 = s.exact_vortex(0.0)
+    err = s.l2_state_error(U0)
+    s.close()
+    out = {"workload": "8x8 cells (128 triangles), 10x10 domain, p=3, integration_order 8, 10000 RK4 steps of dt %.12g"
+                       % dt, "gpu_s": gpu_s, "l2_state_error": [float(x) for x in err]}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        if os.path.exists(O.REF_RUN):
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            with tempfile.TemporaryDirectory() as d:
+                r = O.run_reference(d, prm, prm["number_time_steps"], threads=threads, no_state=True,
+                                    env_extra={"OMP_PROC_BIND": "close", "OMP_PLACES": "cores", "OMP_SCHEDULE": "static"})
+            cpu_s = float(re.search(r"stepping_sec=([0-9.]+)", r["stdout"]).group(1))
+            out.update({"cpu_reference_s": cpu_s, "cpu_threads": threads, "speedup": cpu_s / gpu_s})
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -441,6 +479,8 @@ def main():
         line["shard"] = shard_figure(vt, 200)
     if not a.no_pcie:
         line["pcie_inclusive"] = pcie_figure(vt, 5)
+    if not a.no_c1:
+        line["c1_full_run"] = c1_figure(vt, not a.no_cpu)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mesh, p, order, a.cpu_steps)
     print(json.dumps(line))
