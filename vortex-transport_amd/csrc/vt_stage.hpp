@@ -68,27 +68,30 @@ __device__ __forceinline__ double fast_rcp(double x) {
 // Roe flux of Numericalflux.cpp:22-166 with the reference's eigenvalue
 // "sound speed" ci = c2^2 (Numericalflux.cpp:117).  Divisions are taken as
 // refined reciprocals (one per density, 1/(1+di), 1/ci, and 1/eps = 10/ci),
-// and the pressure uses 0.5 rho q^2 = 0.5 (m_x^2 + m_y^2) / rho; all are
-// reassociations of the reference arithmetic at the ulp level.  Returns 0 or
-// the reference's error number.
+// the pressure as (gamma-1) rho E - (gamma-1)/2 (m_x u + m_y v), the entropy
+// fix's |l| < eps as one comparison of |l|, and the flux assembly with its
+// halves and common factors folded (one multiply by 1/2 per component, C1 and
+// C2 sharing G1/ci); all are reassociations of the reference arithmetic at the
+// ulp level.  Returns 0 or the reference's error number.
 __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR)[4], double nx, double ny,
                                         double (&F)[4]) {
-    constexpr double gamma = 1.4;
-    constexpr double gmi = gamma - 1.0;
+    constexpr double gmi = 0.4;
+    // States: u = m/rho, p = (gamma-1)(rho E - (m_x u + m_y v)/2), H = (rho E + p)/rho.
     const double rL = UL[0], irL = fast_rcp(rL);
     const double uL = UL[1] * irL, vL = UL[2] * irL;
     const double unL = uL * nx + vL * ny;
-    const double pL = gmi * (UL[3] - 0.5 * (UL[1] * UL[1] + UL[2] * UL[2]) * irL);
+    const double pL = fma(-0.5 * gmi, fma(UL[2], vL, UL[1] * uL), gmi * UL[3]);
     const double rR = UR[0], irR = fast_rcp(rR);
     const double uR = UR[1] * irR, vR = UR[2] * irR;
     const double unR = uR * nx + vR * ny;
-    const double pR = gmi * (UR[3] - 0.5 * (UR[1] * UR[1] + UR[2] * UR[2]) * irR);
+    const double pR = fma(-0.5 * gmi, fma(UR[2], vR, UR[1] * uR), gmi * UR[3]);
     int code = 0;
     if ((pL <= 0) || (rL <= 0)) code = 1;
     else if ((pR <= 0) || (rR <= 0)) code = 2;
     const double rHL = UL[3] + pL, HL = rHL * irL;
     const double rHR = UR[3] + pR, HR = rHR * irR;
     const double du0 = UR[0] - UL[0], du1 = UR[1] - UL[1], du2 = UR[2] - UL[2], du3 = UR[3] - UL[3];
+    // Roe averages.
     const double di = sqrt(rR * irL);
     const double d1 = fast_rcp(1.0 + di);
     const double ui = (di * uR + uL) * d1;
@@ -98,24 +101,31 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
     const double ucp = ui * nx + vi * ny;
     const double c2 = gmi * (Hi - af);
     if (code == 0 && c2 <= 0) code = 3;
+    // Eigenvalues with the reference's "sound speed" ci = c2^2 and its entropy
+    // fix: |l| < eps (eps = ci/10) is replaced by (eps + l^2/eps)/2.
     const double ci = c2 * c2;
     const double ci1 = fast_rcp(ci);
-    const double eps = ci * 0.1, ieps = 10.0 * ci1;
+    const double eps = ci * 0.1, heps = ci * 0.05, hieps = 5.0 * ci1;
     double l0 = ucp + ci, l1 = ucp - ci, l2 = ucp;
-    if ((l0 < eps) && (l0 > -eps)) l0 = 0.5 * (eps + l0 * l0 * ieps);
-    if ((l1 < eps) && (l1 > -eps)) l1 = 0.5 * (eps + l1 * l1 * ieps);
-    if ((l2 < eps) && (l2 > -eps)) l2 = 0.5 * (eps + l2 * l2 * ieps);
+    if (fabs(l0) < eps) l0 = fma(l0 * l0, hieps, heps);
+    if (fabs(l1) < eps) l1 = fma(l1 * l1, hieps, heps);
+    if (fabs(l2) < eps) l2 = fma(l2 * l2, hieps, heps);
     l0 = fabs(l0); l1 = fabs(l1); l2 = fabs(l2);
     const double s1 = 0.5 * (l0 + l1);
     const double s2 = 0.5 * (l0 - l1);
+    const double sl = s1 - l2;
     const double G1 = gmi * (af * du0 - ui * du1 - vi * du2 + du3);
     const double G2 = -ucp * du0 + du1 * nx + du2 * ny;
-    const double C1 = G1 * (s1 - l2) * ci1 * ci1 + G2 * s2 * ci1;
-    const double C2 = G1 * s2 * ci1 + G2 * (s1 - l2);
-    F[0] = 0.5 * (rL * unL + rR * unR) - 0.5 * (l2 * du0 + C1);
-    F[1] = 0.5 * ((UL[1] * unL + pL * nx) + (UR[1] * unR + pR * nx)) - 0.5 * (l2 * du1 + C1 * ui + C2 * nx);
-    F[2] = 0.5 * ((UL[2] * unL + pL * ny) + (UR[2] * unR + pR * ny)) - 0.5 * (l2 * du2 + C1 * vi + C2 * ny);
-    F[3] = 0.5 * (rHL * unL + rHR * unR) - 0.5 * (l2 * du3 + C1 * Hi + C2 * ucp);
+    // C1 = G1 (s1 - l2) ci1^2 + G2 s2 ci1,  C2 = G1 s2 ci1 + G2 (s1 - l2)
+    const double a1 = G1 * ci1;
+    const double C1 = ci1 * fma(a1, sl, G2 * s2);
+    const double C2 = fma(a1, s2, G2 * sl);
+    // F = (F_L.n + F_R.n)/2 - (l2 du + C1 (1, ui, vi, Hi) + C2 (0, nx, ny, ucp))/2
+    const double pp = pL + pR;
+    F[0] = 0.5 * (fma(rL, unL, rR * unR) - fma(l2, du0, C1));
+    F[1] = 0.5 * (fma(UL[1], unL, fma(UR[1], unR, pp * nx)) - fma(l2, du1, fma(C1, ui, C2 * nx)));
+    F[2] = 0.5 * (fma(UL[2], unL, fma(UR[2], unR, pp * ny)) - fma(l2, du2, fma(C1, vi, C2 * ny)));
+    F[3] = 0.5 * (fma(rHL, unL, rHR * unR) - fma(l2, du3, fma(C1, Hi, C2 * ucp)));
     return code;
 }
 
