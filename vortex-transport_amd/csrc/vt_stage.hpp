@@ -65,9 +65,21 @@ __device__ __forceinline__ double fast_rcp(double x) {
     return fma(r, e, r);
 }
 
+// Reciprocal square root from the hardware estimate (v_rsq_f64) refined by
+// two Newton steps (positive normal operands, as fast_rcp).
+__device__ __forceinline__ double fast_rsq(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
 // Roe flux of Numericalflux.cpp:22-166 with the reference's eigenvalue
-// "sound speed" ci = c2^2 (Numericalflux.cpp:117).  Divisions are taken as
-// refined reciprocals (one per density, 1/(1+di), 1/ci, and 1/eps = 10/ci),
+// "sound speed" ci = c2^2 (Numericalflux.cpp:117).  1/rho_L, 1/rho_R and the
+// Roe ratio sqrt(rho_R/rho_L) come from one refined 1/sqrt(rho_L rho_R)
+// (within 4 ulp of the divisions, 2 ulp of the square root), the other
+// divisions are refined reciprocals (1/(1+di), 1/ci, and 1/eps = 10/ci),
 // the pressure as (gamma-1) rho E - (gamma-1)/2 (m_x u + m_y v), the entropy
 // fix's |l| < eps as one comparison of |l|, and the flux assembly with its
 // halves and common factors folded (one multiply by 1/2 per component, C1 and
@@ -77,11 +89,25 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
                                         double (&F)[4]) {
     constexpr double gmi = 0.4;
     // States: u = m/rho, p = (gamma-1)(rho E - (m_x u + m_y v)/2), H = (rho E + p)/rho.
-    const double rL = UL[0], irL = fast_rcp(rL);
+    // One refined reciprocal square root of rho_L rho_R gives both
+    // reciprocals and the Roe ratio sqrt(rho_R / rho_L).  When the densities
+    // differ in sign or one is zero (a state the checks below report), each
+    // side is taken on its own, so that each side's check sees its own values.
+    const double rL = UL[0], rR = UR[0], prd = rL * rR;
+    double irL, irR, di;
+    if (prd > 0) {
+        const double t = fast_rsq(prd), tt = t * t;
+        irL = rR * tt;
+        irR = rL * tt;
+        di = rR * t;
+    } else {
+        irL = fast_rcp(rL);
+        irR = fast_rcp(rR);
+        di = sqrt(rR * irL);
+    }
     const double uL = UL[1] * irL, vL = UL[2] * irL;
     const double unL = uL * nx + vL * ny;
     const double pL = fma(-0.5 * gmi, fma(UL[2], vL, UL[1] * uL), gmi * UL[3]);
-    const double rR = UR[0], irR = fast_rcp(rR);
     const double uR = UR[1] * irR, vR = UR[2] * irR;
     const double unR = uR * nx + vR * ny;
     const double pR = fma(-0.5 * gmi, fma(UR[2], vR, UR[1] * uR), gmi * UR[3]);
@@ -92,7 +118,6 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
     const double rHR = UR[3] + pR, HR = rHR * irR;
     const double du0 = UR[0] - UL[0], du1 = UR[1] - UL[1], du2 = UR[2] - UL[2], du3 = UR[3] - UL[3];
     // Roe averages.
-    const double di = sqrt(rR * irL);
     const double d1 = fast_rcp(1.0 + di);
     const double ui = (di * uR + uL) * d1;
     const double vi = (di * vR + vL) * d1;
