@@ -264,8 +264,9 @@ __global__ void __launch_bounds__(kAdjBlock, VT_ADJ_MIN_WAVES) adjoint_stage_ker
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         const double dx = vx[(s + 1) % 3] - vx[s], dy = vy[(s + 1) % 3] - vy[s];
-        const double len = sqrt(dx * dx + dy * dy);
-        const double nxv = dy / len, nyv = -dx / len;
+        // |side| and the unit normal from one refined 1/|side|.
+        const double l2s = dx * dx + dy * dy, il = fast_rsq(l2s);
+        const double len = l2s * il, nxv = dy * il, nyv = -dx * il;
         const ctab_t tp = T + tab_tp(P) + s * NQ * NE;
         const ctab_t tm = T + tab_tm(P, NQ) + s * NQ * NE;
         const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;

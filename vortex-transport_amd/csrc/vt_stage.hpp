@@ -293,8 +293,9 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         const int zv = opaque_zero();
         const double dx = plane_at(prm.vtx, 2 * ((s + 1) % 3) + zv, ld8, oe) - plane_at(prm.vtx, 2 * s + zv, ld8, oe);
         const double dy = plane_at(prm.vtx, 2 * ((s + 1) % 3) + 1 + zv, ld8, oe) - plane_at(prm.vtx, 2 * s + 1 + zv, ld8, oe);
-        const double len = sqrt(dx * dx + dy * dy);
-        const double nxv = dy / len, nyv = -dx / len;
+        // |side| and the unit normal from one refined 1/|side|.
+        const double l2s = dx * dx + dy * dy, il = fast_rsq(l2s);
+        const double len = l2s * il, nxv = dy * il, nyv = -dx * il;
         const ctab_t tp = T + tab_tp(P) + s * NQ * NE;
         const ctab_t tm = T + tab_tm(P, NQ) + s * NQ * NE;
         const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;
