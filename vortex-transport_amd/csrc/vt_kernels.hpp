@@ -64,7 +64,7 @@ struct BatchParams {
 // Yout; kAdj4..kAdj1 are the four stages of one reverse RK4 step; kAdjSeed
 // writes Yout = scale * Minv_ref^T lam / detJ (no flux work).
 enum AdjMode : int { kAdjRaw = 0, kAdj1 = 1, kAdj2 = 2, kAdj3 = 3, kAdj4 = 4, kAdjSeed = 5 };
-constexpr int kAdjBlock = 256;
+constexpr int kAdjBlock = 64;
 
 struct AdjParams {
     const double* U;      // stage state the right-hand side is linearised at
