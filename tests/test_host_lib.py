@@ -167,15 +167,15 @@ def test_batch_adjoint_entry_points_reject_null_handles(vt):
 
 
 def test_sizes_beyond_the_offset_limit_are_refused_before_any_array_is_read(vt):
-    # 32-bit byte offsets within a plane set: n_e * 4 * Np * 8 B < 4 GiB, i.e.
-    # n_e below ~13.4 M at p = 3.  The limit is checked before the arrays are
+    # At most 2^28 elements per context (element byte offsets and neighbour
+    # indices stay below 2^31).  The limit is checked before the arrays are
     # touched, so one-element stand-ins suffice (nothing reaches the GPU).
     L = vt.lib()
     v = np.zeros(6)
     b = np.zeros(3, dtype=np.int32)
     t = np.zeros(1, dtype=np.int32)
     h = vt.ctypes.c_void_p()
-    assert L.vt_create(14_000_000, vt._d(v), vt._i(b), vt._i(t), 3, 8, vt.ctypes.byref(h)) == vt.VT_ERR_ARG
+    assert L.vt_create(300_000_000, vt._d(v), vt._i(b), vt._i(t), 3, 8, vt.ctypes.byref(h)) == vt.VT_ERR_ARG
     assert not h.value
     n = np.array([1_500_000_000, 1_000_000_000], dtype=np.int32)
     pp = np.array([3, 3], dtype=np.int32)

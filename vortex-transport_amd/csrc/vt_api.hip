@@ -272,11 +272,8 @@ namespace vti {
 
 int create_ctx(int n, const double* vertices, const int* neighbors, const int* type, int p, const vt::OperatorTable& tab,
                vt_ctx** out) {
-    // Size limits first (nothing of the arrays is read for a refused size):
-    // 32-bit byte offsets within a plane set and the 40-bit error key.
-    if (static_cast<long long>(plane_stride(n)) * 8 * 4 * vt::num_nodes(p) >= (1ll << 32) ||
-        static_cast<long long>(n) * 256 >= (1ll << 40))
-        return VT_ERR_ARG;
+    // Size limit first (nothing of the arrays is read for a refused size).
+    if (n > kMaxElements) return VT_ERR_ARG;
     for (int e = 0; e < n; ++e) {
         if (type[e] != 0 && type[e] != 1) return VT_ERR_ARG;
         for (int s = 0; s < 3; ++s)

@@ -252,8 +252,8 @@ int vt_batch_create(int n_inst, const int* n_elem, const int* p, const int* orde
         first[i + 1] = first[i] + n_elem[i];
     }
     // Whole-ensemble size limit (int element indices, 40-bit error key) before
-    // any array is read; each group's plane-offset limit is checked when it is
-    // created.
+    // any array is read; each group's own limit (kMaxElements) is checked when
+    // it is created.
     if (first[n_inst] >= (1ll << 31) || first[n_inst] * 256 >= (1ll << 40)) return VT_ERR_ARG;
     for (int i = 0; i < n_inst; ++i) {
         for (long long e = first[i]; e < first[i + 1]; ++e)

@@ -54,6 +54,13 @@ bool hip_ok(hipError_t e, const char* what);
 // 4*Np planes of one element do not all start at the same HBM channel offset.
 inline int plane_stride(int ne) { return ((ne + 255) / 256) * 256 + 64; }
 
+// Largest mesh of one context: kernels address an element within a plane by
+// a 32-bit byte offset (e * 8) and the neighbour table by the int s * ld + e;
+// plane bases are 64-bit, so a plane set may exceed 4 GiB.  2^28 elements keep
+// both offsets below 2^31 (the device memory runs out first: 86 GB per state
+// at p = 3).
+constexpr int kMaxElements = 1 << 28;
+
 // Graph steps per replay: one graph covers up to kGraphSteps steps.
 constexpr int kGraphSteps = 64;
 
