@@ -101,6 +101,7 @@ vt::StageParams base_params(vt_ctx* c) {
     prm.ne = c->ne;
     prm.ld = c->ld;
     prm.blocks_per_xcd = vt::tiles_per_xcd(c->ne, c->cfg.block);
+    prm.pairs = c->pairs;
     return prm;
 }
 
@@ -298,6 +299,12 @@ int create_ctx(int n, const double* vertices, const int* neighbors, const int* t
     std::vector<double> hv(static_cast<size_t>(6) * c->ld, 0.0);
     std::vector<int> ns(static_cast<size_t>(3) * n), hn(static_cast<size_t>(3) * c->ld, 0);
     vt::side_neighbours(n, neighbors, type, ns.data());
+    // The reference's meshes number the two triangles of a cell 2c, 2c + 1 and
+    // put their common diagonal on side s = 1 of both; the stage kernel then
+    // evaluates each diagonal once (half of its points on each side).
+    c->pairs = n % 2 == 0 ? 1 : 0;
+    for (int e = 0; e < n && c->pairs; ++e)
+        if (ns[3 * static_cast<size_t>(e) + 1] != (e ^ 1)) c->pairs = 0;
     for (int e = 0; e < n; ++e) {
         for (int k = 0; k < 6; ++k) hv[static_cast<size_t>(k) * c->ld + e] = vertices[6 * static_cast<size_t>(e) + k];
         for (int s = 0; s < 3; ++s) hn[static_cast<size_t>(s) * c->ld + e] = ns[3 * static_cast<size_t>(e) + s];

@@ -75,6 +75,7 @@ hipError_t enqueue_stage(vt_batch* b, double dt, int mode, int stage_id, Bufs bu
             prm.mode = mode;
             prm.stage_id = stage_id;
             prm.err_base = static_cast<int>(b->gbase[L[i]]);
+            prm.pairs = c->pairs;
             bp.p[i] = c->p;
             bp.tiles[i] = (c->ne + b->cfg.block - 1) / b->cfg.block;
             bp.tpx[i] = (bp.tiles[i] + 7) / 8;

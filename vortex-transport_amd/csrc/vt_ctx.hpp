@@ -13,6 +13,7 @@
 
 struct vt_ctx {
     int ne = 0, ld = 0, p = 0, np = 0, nq = 0, order = 0, device = 0;
+    int pairs = 0;                           // side s = 1 of every element e faces e ^ 1
     hipStream_t stream = nullptr;
     double *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr, *vtx = nullptr, *tab = nullptr;
     int* nbr = nullptr;

@@ -33,6 +33,7 @@ struct StageParams {
     int ne, ld, mode, stage_id;
     int blocks_per_xcd;   // tiles per XCD band: ceil(ceil(ne / block) / 8)
     int err_base;         // added to the element index of an error report
+    int pairs;            // 1: the neighbour across side s = 1 of every element e is e ^ 1
 };
 
 // Launch shape.  block: elements per workgroup (128 or 256).  persistent:

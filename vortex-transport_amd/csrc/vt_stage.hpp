@@ -84,9 +84,11 @@ __device__ __forceinline__ double fast_rsq(double x) {
 // fix's |l| < eps as one comparison of |l|, and the flux assembly with its
 // halves and common factors folded (one multiply by 1/2 per component, C1 and
 // C2 sharing G1/ci); all are reassociations of the reference arithmetic at the
-// ulp level.  Returns 0 or the reference's error number.
-__device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR)[4], double nx, double ny,
-                                        double (&F)[4]) {
+// ulp level.  Returns the checks' outcome as bits: 1 left state non-physical,
+// 2 right state non-physical, 4 c2 <= 0 (flux_code turns them into the
+// reference's error number).
+__device__ __forceinline__ int roe_flux_bits(const double (&UL)[4], const double (&UR)[4], double nx, double ny,
+                                             double (&F)[4]) {
     constexpr double gmi = 0.4;
     // States: u = m/rho, p = (gamma-1)(rho E - (m_x u + m_y v)/2), H = (rho E + p)/rho.
     // One refined reciprocal square root of rho_L rho_R gives both
@@ -111,9 +113,8 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
     const double uR = UR[1] * irR, vR = UR[2] * irR;
     const double unR = uR * nx + vR * ny;
     const double pR = fma(-0.5 * gmi, fma(UR[2], vR, UR[1] * uR), gmi * UR[3]);
-    int code = 0;
-    if ((pL <= 0) || (rL <= 0)) code = 1;
-    else if ((pR <= 0) || (rR <= 0)) code = 2;
+    int bits = ((pL <= 0) || (rL <= 0)) ? 1 : 0;
+    if ((pR <= 0) || (rR <= 0)) bits |= 2;
     const double rHL = UL[3] + pL, HL = rHL * irL;
     const double rHR = UR[3] + pR, HR = rHR * irR;
     const double du0 = UR[0] - UL[0], du1 = UR[1] - UL[1], du2 = UR[2] - UL[2], du3 = UR[3] - UL[3];
@@ -125,7 +126,7 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
     const double af = 0.5 * (ui * ui + vi * vi);
     const double ucp = ui * nx + vi * ny;
     const double c2 = gmi * (Hi - af);
-    if (code == 0 && c2 <= 0) code = 3;
+    if (c2 <= 0) bits |= 4;
     // Eigenvalues with the reference's "sound speed" ci = c2^2 and its entropy
     // fix: |l| < eps (eps = ci/10) is replaced by (eps + l^2/eps)/2.
     const double ci = c2 * c2;
@@ -151,7 +152,35 @@ __device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR
     F[1] = 0.5 * (fma(UL[1], unL, fma(UR[1], unR, pp * nx)) - fma(l2, du1, fma(C1, ui, C2 * nx)));
     F[2] = 0.5 * (fma(UL[2], unL, fma(UR[2], unR, pp * ny)) - fma(l2, du2, fma(C1, vi, C2 * ny)));
     F[3] = 0.5 * (fma(rHL, unL, rHR * unR) - fma(l2, du3, fma(C1, Hi, C2 * ucp)));
-    return code;
+    return bits;
+}
+
+// The reference's error number of Numericalflux.cpp:42-53, 75-78, 111-114:
+// the left state is checked first, then the right one, then c2.
+__device__ __forceinline__ int flux_code(int bits) {
+    return (bits & 1) ? 1 : (bits & 2) ? 2 : (bits & 4) ? 3 : 0;
+}
+
+// The same outcome seen from the other side of the edge (left and right swapped).
+__device__ __forceinline__ int swap_sides(int bits) {
+    return ((bits & 1) << 1) | ((bits >> 1) & 1) | (bits & 4);
+}
+
+// Returns 0 or the reference's error number.
+__device__ __forceinline__ int roe_flux(const double (&UL)[4], const double (&UR)[4], double nx, double ny,
+                                        double (&F)[4]) {
+    return flux_code(roe_flux_bits(UL, UR, nx, ny, F));
+}
+
+// Value of the neighbouring lane (lane ^ 1) of the wavefront, by a DPP
+// quad permutation [1, 0, 3, 2]; both lanes of each pair must be active.
+__device__ __forceinline__ int lane_pair_swap(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ double lane_pair_swap(double x) {
+    const int lo = lane_pair_swap(__double2loint(x));
+    const int hi = lane_pair_swap(__double2hiint(x));
+    return __hiloint2double(hi, lo);
 }
 
 // Operator tables are read through the constant address space: no kernel
@@ -299,8 +328,24 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         const ctab_t tp = T + tab_tp(P) + s * NQ * NE;
         const ctab_t tm = T + tab_tm(P, NQ) + s * NQ * NE;
         const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;
+        auto lift = [&](int q, const double (&F)[4]) {
+#pragma unroll
+            for (int m = 0; m < NE; ++m) {
+                const double w = len * tl[q * NE + m];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) r[edge_node<P>(s, m)][k] -= w * F[k];
+            }
+        };
+        // On a paired mesh side s = 1 is the diagonal of the element's cell,
+        // shared with lane ^ 1: each of the two evaluates the points q < ceil(NQ/2) of
+        // its own orientation and takes the rest from the other, whose point
+        // NQ-1-q is the same physical point with the normal reversed, by the
+        // flux's antisymmetry F(a, b, n) = -F(b, a, -n).  The other lane's
+        // check outcome comes along, so every element reports its own points.
+        const bool half = (s == 1) && prm.pairs;
+        const int nqe = half ? (NQ + 1) / 2 : NQ;
 #pragma unroll VT_Q_UNROLL
-        for (int q = 0; q < NQ; ++q) {
+        for (int q = 0; q < nqe; ++q) {
             double UL[4], UR[4], F[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -313,14 +358,18 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                 UL[k] = a;
                 UR[k] = b;
             }
-            const int code = roe_flux(UL, UR, nxv, nyv, F);
-            if (code) report(prm.err, prm.stage_id, prm.err_base + e, q, s, code);
+            const int bits = roe_flux_bits(UL, UR, nxv, nyv, F);
+            if (half && q != NQ - 1 - q) {
+                double G[4];
 #pragma unroll
-            for (int m = 0; m < NE; ++m) {
-                const double w = len * tl[q * NE + m];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) r[edge_node<P>(s, m)][k] -= w * F[k];
+                for (int k = 0; k < 4; ++k) G[k] = -lane_pair_swap(F[k]);
+                const int rcode = flux_code(swap_sides(lane_pair_swap(bits)));
+                if (rcode) report(prm.err, prm.stage_id, prm.err_base + e, NQ - 1 - q, s, rcode);
+                lift(NQ - 1 - q, G);
             }
+            const int code = flux_code(bits);
+            if (code) report(prm.err, prm.stage_id, prm.err_base + e, q, s, code);
+            lift(q, F);
         }
         if (s < 2) {
             if (static_cast<unsigned>(loc[s + 1]) < static_cast<unsigned>(BS)) {
