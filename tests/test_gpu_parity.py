@@ -327,3 +327,43 @@ def test_l2_state_error_matches_oracle(vt, oracle, gpu, p):
     if p == 3:
         ref = oracle.l2_state_error(mesh.element_coordinates, U0, s.state())
         assert np.all(np.abs(err - ref) <= 1e-12 * ref), (err, ref)
+
+
+# --- element numbering without the cell pairs -----------------------------------
+
+def _relabel(vt, mesh, shift):
+    """The same mesh with element e renumbered (e + shift) mod n: cell c's two
+    triangles are then no longer 2c', 2c' + 1 for odd shifts."""
+    n = mesh.num_elements
+    pi = (np.arange(n) + shift) % n
+    v = np.empty_like(mesh.element_coordinates)
+    b = np.empty_like(mesh.elements_at_boundary)
+    t = np.empty_like(mesh.element_type)
+    v[pi] = mesh.element_coordinates
+    b[pi] = pi[mesh.elements_at_boundary]
+    t[pi] = mesh.element_type
+    return vt.Mesh(v, b, t), pi
+
+
+@pytest.mark.parametrize("p", [1, 3])
+def test_unpaired_numbering_matches_oracle_and_paired(vt, oracle, gpu, p):
+    # An odd relabelling puts the two triangles of a cell on lanes that are not
+    # l, l ^ 1, so every side is evaluated in full (the diagonal is not split
+    # between the two); both forms agree with the oracle and with each other.
+    mesh = vt.generate_mesh(nx=12, ny=10, perturb_grid=0.0)
+    m2, pi = _relabel(vt, mesh, 1)
+    a = vt.Solver(mesh, p, 8)
+    a.initialize_hydrodinamics(0, 0.3, 0.4, -0.3)
+    c = vt.Solver(m2, p, 8)
+    c.initialize_hydrodinamics(0, 0.3, 0.4, -0.3)
+    U0 = a.state()
+    assert np.array_equal(c.state()[pi], U0)
+    o = oracle.Solver(m2.element_coordinates, m2.elements_at_boundary, m2.element_type, p, 8)
+    o.set_state(c.state())
+    assert o.run(1, 1e-3, 40) == 0
+    a.rk4(1e-3, 40)
+    c.rk4(1e-3, 40)
+    assert rel(c.state(), o.state()) <= 1e-12
+    assert rel(c.state()[pi], a.state()) <= 1e-13
+    a.close()
+    c.close()
