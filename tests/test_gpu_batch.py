@@ -201,3 +201,27 @@ def test_ensemble_64_equals_single_instances(vt, gpu):
         d = draws[i]
         U = _single(vt, meshes[i], (d["cells"], d["p"], 8, d["epsilon"], d["x0"], d["y0"]), 1, dt, steps)
         assert same(b.state(i), U), i
+
+
+def test_batch_groups_with_and_without_cell_pairs(vt, gpu):
+    # One launch over a group numbered as the reference numbers it (cell
+    # diagonals evaluated once, half of their points on each triangle) and a
+    # group whose elements are renumbered (e + 1) mod n (every side in full):
+    # each instance equals its plain run.
+    base = vt.generate_mesh(nx=10, ny=9)
+    n = base.num_elements
+    pi = (np.arange(n) + 1) % n
+    v = np.empty_like(base.element_coordinates)
+    bn = np.empty_like(base.elements_at_boundary)
+    t = np.empty_like(base.element_type)
+    v[pi], bn[pi], t[pi] = base.element_coordinates, pi[base.elements_at_boundary], base.element_type
+    shifted = vt.Mesh(v, bn, t)
+    cfg = [(None, 1, 8, 0.3, 0.5, -0.5), (None, 3, 8, 0.25, -1.0, 1.0)]
+    meshes = [base, shifted]
+    b = vt.Batch(meshes, [c[1] for c in cfg], [c[2] for c in cfg])
+    b.initialize_hydrodinamics(0, [c[3] for c in cfg], [c[4] for c in cfg], [c[5] for c in cfg])
+    assert b.num_launches == 1
+    b.rk4(1e-3, 30)
+    for i, (m, c) in enumerate(zip(meshes, cfg)):
+        assert same(b.state(i), _single(vt, m, c, 1, 1e-3, 30))
+    b.close()
