@@ -72,7 +72,10 @@ int vt_quadrature(int integration_order, int *nq_line, double *line_xw, int *n_a
  * connectivity.  Replaces the element set-up loop of main.cpp:115-147: the
  * Element constructor, build_jacobians, build_mass_matrix_inverse,
  * build_stiffness_matrix (Element.cpp:21-116) and the Evolve_element
- * constructor with evaluate_basis_in_quadrature_poits (Evolve.cpp:24-132). */
+ * constructor with evaluate_basis_in_quadrature_poits (Evolve.cpp:24-132).
+ * Any element numbering is accepted; with the reference's own (the two
+ * triangles of a cell numbered 2c, 2c + 1, Meshgeneration.cpp:63-117) each
+ * cell diagonal's flux is evaluated once for both triangles. */
 int vt_create(int n_elem, const double *vertices, const int *neighbors, const int *type, int p,
               int integration_order, vt_ctx **out);
 /* Same, with the operators the reference passes to each element instead of an
