@@ -328,10 +328,10 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         const ctab_t tp = T + tab_tp(P) + s * NQ * NE;
         const ctab_t tm = T + tab_tm(P, NQ) + s * NQ * NE;
         const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;
-        auto lift = [&](int q, const double (&F)[4]) {
+        auto lift = [&](const double (&tw)[NE], const double (&F)[4]) {
 #pragma unroll
             for (int m = 0; m < NE; ++m) {
-                const double w = len * tl[q * NE + m];
+                const double w = len * tw[m];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) r[edge_node<P>(s, m)][k] -= w * F[k];
             }
@@ -344,6 +344,12 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         // check outcome comes along, so every element reports its own points.
         const bool half = (s == 1) && prm.pairs;
         const int nqe = half ? (NQ + 1) / 2 : NQ;
+        double ctp[NE], ctm[NE];
+#pragma unroll
+        for (int m = 0; m < NE; ++m) {
+            ctp[m] = tp[m];
+            ctm[m] = tm[m];
+        }
 #pragma unroll VT_Q_UNROLL
         for (int q = 0; q < nqe; ++q) {
             double UL[4], UR[4], F[4];
@@ -352,11 +358,23 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                 double a = 0.0, b = 0.0;
 #pragma unroll
                 for (int m = 0; m < NE; ++m) {
-                    a += tp[q * NE + m] * my[(k * NP + edge_node<P>(s, m)) * BS];
-                    b += tm[q * NE + m] * un[m][k];
+                    a += ctp[m] * my[(k * NP + edge_node<P>(s, m)) * BS];
+                    b += ctm[m] * un[m][k];
                 }
                 UL[k] = a;
                 UR[k] = b;
+            }
+            // Table values of the lifts of this point and of the next point's
+            // traces, requested before the flux so that their scalar loads are
+            // in flight during it.
+            const int qn = q + 1 < nqe ? q + 1 : q;
+            double ctl[NE], crl[NE];
+#pragma unroll
+            for (int m = 0; m < NE; ++m) {
+                ctp[m] = tp[qn * NE + m];
+                ctm[m] = tm[qn * NE + m];
+                ctl[m] = tl[q * NE + m];
+                crl[m] = tl[(NQ - 1 - q) * NE + m];
             }
             const int bits = roe_flux_bits(UL, UR, nxv, nyv, F);
             if (half && q != NQ - 1 - q) {
@@ -365,11 +383,11 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                 for (int k = 0; k < 4; ++k) G[k] = -lane_pair_swap(F[k]);
                 const int rcode = flux_code(swap_sides(lane_pair_swap(bits)));
                 if (rcode) report(prm.err, prm.stage_id, prm.err_base + e, NQ - 1 - q, s, rcode);
-                lift(NQ - 1 - q, G);
+                lift(crl, G);
             }
             const int code = flux_code(bits);
             if (code) report(prm.err, prm.stage_id, prm.err_base + e, q, s, code);
-            lift(q, F);
+            lift(ctl, F);
         }
         if (s < 2) {
             if (static_cast<unsigned>(loc[s + 1]) < static_cast<unsigned>(BS)) {
