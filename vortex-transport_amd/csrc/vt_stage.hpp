@@ -204,14 +204,32 @@ __device__ __forceinline__ void report(unsigned long long* err, int stage, unsig
     atomicMin(err, key);
 }
 
-// Plane access: the plane base is wavefront-uniform (scalar registers) and
-// the element offset a 32-bit byte offset, so loads and stores take the
-// global_{load,store} saddr + voffset form instead of 64-bit per-lane addresses.
+// Plane access: element e's value of plane `plane` at byte offset
+// plane * ld8 + e * 8 from base, with the element offset a 32-bit byte offset.
 __device__ __forceinline__ const double& plane_at(const double* base, int plane, size_t ld8, unsigned off8) {
     return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + static_cast<size_t>(plane) * ld8 + off8);
 }
 __device__ __forceinline__ double& plane_at(double* base, int plane, size_t ld8, unsigned off8) {
     return *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + static_cast<size_t>(plane) * ld8 + off8);
+}
+
+// plane_at with the plane base kept in scalar registers: an empty asm hides
+// base + plane * ld8 from the compiler, which otherwise folds the uniform plane
+// offset into a per-lane 64-bit address (one 64-bit multiply-add per access),
+// so loads take the global saddr + 32-bit voffset form (the nontemporal
+// builtins still form a 64-bit address, with one 64-bit add).  base and plane
+// must be wavefront-uniform; a divergent one fails to compile (the asm operand
+// is a scalar register).
+typedef __attribute__((address_space(1))) char* gbyte_t;
+__device__ __forceinline__ const double& uplane_at(const double* base, int plane, size_t ld8, unsigned off8) {
+    gbyte_t pb = (gbyte_t)(const_cast<double*>(base)) + static_cast<size_t>(plane) * ld8;
+    asm("" : "+s"(pb));
+    return *(const double*)(pb + off8);
+}
+__device__ __forceinline__ double& uplane_at(double* base, int plane, size_t ld8, unsigned off8) {
+    gbyte_t pb = (gbyte_t)base + static_cast<size_t>(plane) * ld8;
+    asm("" : "+s"(pb));
+    return *(double*)(pb + off8);
 }
 
 // One tile of BS consecutive elements: BS threads, one element each.  `su`
@@ -246,10 +264,10 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     double un[NE][4];
     if (valid) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) my[v * BS] = plane_at(Uin, v, ld8, oe);
-        x0 = plane_at(prm.vtx, 0, ld8, oe); y0 = plane_at(prm.vtx, 1, ld8, oe);
-        x1 = plane_at(prm.vtx, 2, ld8, oe); y1 = plane_at(prm.vtx, 3, ld8, oe);
-        x2 = plane_at(prm.vtx, 4, ld8, oe); y2 = plane_at(prm.vtx, 5, ld8, oe);
+        for (int v = 0; v < NV; ++v) my[v * BS] = uplane_at(Uin, v, ld8, oe);
+        x0 = uplane_at(prm.vtx, 0, ld8, oe); y0 = uplane_at(prm.vtx, 1, ld8, oe);
+        x1 = uplane_at(prm.vtx, 2, ld8, oe); y1 = uplane_at(prm.vtx, 3, ld8, oe);
+        x2 = uplane_at(prm.vtx, 4, ld8, oe); y2 = uplane_at(prm.vtx, 5, ld8, oe);
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
             const int nb = prm.nbr[s * prm.ld + e];
@@ -261,7 +279,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
 #pragma unroll
             for (int m = 0; m < NE; ++m)
 #pragma unroll
-                for (int k = 0; k < 4; ++k) un[m][k] = plane_at(Uin, k * NP + edge_node<P>(0, m), ld8, onb[0]);
+                for (int k = 0; k < 4; ++k) un[m][k] = uplane_at(Uin, k * NP + edge_node<P>(0, m), ld8, onb[0]);
         }
     }
     __syncthreads();
@@ -317,11 +335,11 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
 #pragma unroll
             for (int m = 0; m < NE; ++m)
 #pragma unroll
-                for (int k = 0; k < 4; ++k) nxt[m][k] = plane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
+                for (int k = 0; k < 4; ++k) nxt[m][k] = uplane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
         }
         const int zv = opaque_zero();
-        const double dx = plane_at(prm.vtx, 2 * ((s + 1) % 3) + zv, ld8, oe) - plane_at(prm.vtx, 2 * s + zv, ld8, oe);
-        const double dy = plane_at(prm.vtx, 2 * ((s + 1) % 3) + 1 + zv, ld8, oe) - plane_at(prm.vtx, 2 * s + 1 + zv, ld8, oe);
+        const double dx = uplane_at(prm.vtx, 2 * ((s + 1) % 3) + zv, ld8, oe) - uplane_at(prm.vtx, 2 * s + zv, ld8, oe);
+        const double dy = uplane_at(prm.vtx, 2 * ((s + 1) % 3) + 1 + zv, ld8, oe) - uplane_at(prm.vtx, 2 * s + 1 + zv, ld8, oe);
         // |side| and the unit normal from one refined 1/|side|.
         const double l2s = dx * dx + dy * dy, il = fast_rsq(l2s);
         const double len = l2s * il, nxv = dy * il, nyv = -dx * il;
@@ -400,7 +418,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                 for (int m = 0; m < NE; ++m)
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
-                        un[m][k] = VT_PREFETCH_NEXT ? nxt[m][k] : plane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
+                        un[m][k] = VT_PREFETCH_NEXT ? nxt[m][k] : uplane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
             }
         }
     }
@@ -438,13 +456,13 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         for (int h = 0; h < 2; ++h) {
             double ac[2 * NP], u0[2 * NP];
 #pragma unroll
-            for (int t = 0; t < 2 * NP; ++t) ac[t] = __builtin_nontemporal_load(&plane_at(prm.acc, 2 * h * NP + t, ld8, oe));
+            for (int t = 0; t < 2 * NP; ++t) ac[t] = __builtin_nontemporal_load(&uplane_at(prm.acc, 2 * h * NP + t, ld8, oe));
             if (mode == kModeRk2) {
 #pragma unroll
                 for (int t = 0; t < 2 * NP; ++t) u0[t] = my[(2 * h * NP + t) * BS] - ac[t] * 0.5;
             } else {
 #pragma unroll
-                for (int t = 0; t < 2 * NP; ++t) u0[t] = __builtin_nontemporal_load(&plane_at(prm.Un, 2 * h * NP + t, ld8, oe));
+                for (int t = 0; t < 2 * NP; ++t) u0[t] = __builtin_nontemporal_load(&uplane_at(prm.Un, 2 * h * NP + t, ld8, oe));
             }
             // Node-major, so each row of Minv is read once per half and dies; the
             // table base is made opaque per half so that its loads are not
@@ -458,11 +476,11 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                     const int k = 2 * h + kk2, t = kk2 * NP + i, v = k * NP + i;
                     const double kk = dt * dudt_m(M, i, k);
                     if (mode == kModeRk2) {
-                        __builtin_nontemporal_store(u0[t] + kk * 0.5, &plane_at(prm.Uout, v, ld8, oe));
+                        __builtin_nontemporal_store(u0[t] + kk * 0.5, &uplane_at(prm.Uout, v, ld8, oe));
                     } else {
                         const double k2 = 2.0 * (my[v * BS] - u0[t]);
-                        __builtin_nontemporal_store(u0[t] + c6 * ((ac[t] + 2.0 * k2) + 2.0 * kk), &plane_at(prm.acc, v, ld8, oe));
-                        __builtin_nontemporal_store(u0[t] + kk, &plane_at(prm.Uout, v, ld8, oe));
+                        __builtin_nontemporal_store(u0[t] + c6 * ((ac[t] + 2.0 * k2) + 2.0 * kk), &uplane_at(prm.acc, v, ld8, oe));
+                        __builtin_nontemporal_store(u0[t] + kk, &uplane_at(prm.Uout, v, ld8, oe));
                     }
                 }
             __builtin_amdgcn_sched_barrier(0);
@@ -470,13 +488,13 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     } else if (mode == kModeRk4) {  // U_{n+1} = acc + k4/6
         double ac[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) ac[v] = __builtin_nontemporal_load(&plane_at(prm.acc, v, ld8, oe));
+        for (int v = 0; v < NV; ++v) ac[v] = __builtin_nontemporal_load(&uplane_at(prm.acc, v, ld8, oe));
 #pragma unroll
         for (int i = 0; i < NP; ++i)
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int v = k * NP + i;
-                __builtin_nontemporal_store(ac[v] + c6 * (dt * dudt(i, k)), &plane_at(prm.Uout, v, ld8, oe));
+                __builtin_nontemporal_store(ac[v] + c6 * (dt * dudt(i, k)), &uplane_at(prm.Uout, v, ld8, oe));
             }
     } else if (mode == kModeRk1) {  // k1 = dt f(Un); acc = k1; U2 = Un + k1/2
 #pragma unroll
@@ -485,8 +503,8 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
             for (int k = 0; k < 4; ++k) {
                 const int v = k * NP + i;
                 const double kk = dt * dudt(i, k);
-                __builtin_nontemporal_store(kk, &plane_at(prm.acc, v, ld8, oe));
-                __builtin_nontemporal_store(my[v * BS] + kk * 0.5, &plane_at(prm.Uout, v, ld8, oe));
+                __builtin_nontemporal_store(kk, &uplane_at(prm.acc, v, ld8, oe));
+                __builtin_nontemporal_store(my[v * BS] + kk * 0.5, &uplane_at(prm.Uout, v, ld8, oe));
             }
     } else if (mode == kModeEuler) {  // U + dt dU/dt
 #pragma unroll
@@ -494,13 +512,13 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int v = k * NP + i;
-                __builtin_nontemporal_store(my[v * BS] + dt * dudt(i, k), &plane_at(prm.Uout, v, ld8, oe));
+                __builtin_nontemporal_store(my[v * BS] + dt * dudt(i, k), &uplane_at(prm.Uout, v, ld8, oe));
             }
     } else {  // kModeRhs
 #pragma unroll
         for (int i = 0; i < NP; ++i)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(dudt(i, k), &plane_at(prm.Uout, k * NP + i, ld8, oe));
+            for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(dudt(i, k), &uplane_at(prm.Uout, k * NP + i, ld8, oe));
     }
 }
 
