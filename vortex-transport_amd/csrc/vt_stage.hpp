@@ -530,7 +530,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
 // row away) of the band's rows are read through that XCD's own L2.
 // Occupancy: p = 2 is held to 168 VGPRs for 3 waves per SIMD (a few spills,
 // 2.5% faster than 2 waves); p = 0, 1 reach 3 or more waves as compiled, p = 3
-// runs 2 waves (234 VGPRs).
+// runs 2 waves (232 VGPRs).
 template <int P, int NQ, int BS>
 __global__ void __launch_bounds__(BS, (P == 2 ? 3 : VT_MIN_WAVES)) stage_kernel(StageParams prm) {
     __shared__ double su[4 * ((P + 1) * (P + 2) / 2) * BS];
