@@ -237,6 +237,13 @@ int vt_batch_adjoint(vt_batch *b, double dt, int nsteps, int every, const double
  * vt_vortex_tangent contracted with dJ_i/dU0_i). */
 int vt_batch_vortex_gradient(const vt_batch *b, double t, const double *epsilon, const double *x0, const double *y0,
                              const double *dJdU0, double *grad);
+/* vt_batch_adjoint followed by vt_batch_vortex_gradient without dJdU0 leaving
+ * the device: the objectives J [n_inst] and grad [n_inst][3] = dJ_i/d(epsilon_i,
+ * x0_i, y0_i) of type 0 vortex initial states set at the batch's time at the
+ * call, the tangent contracted with dJ_i/dU0_i on the device (block sums added
+ * in block order).  Same memory and errors as vt_batch_adjoint. */
+int vt_batch_adjoint_vortex(vt_batch *b, double dt, int nsteps, int every, const double *rho_target,
+                            const double *epsilon, const double *x0, const double *y0, double *J, double *grad);
 
 #ifdef __cplusplus
 }

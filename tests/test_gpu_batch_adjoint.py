@@ -1,5 +1,5 @@
 """Ensemble adjoint (vt_batch_adjoint, vt_batch_mass_l2_objective,
-vt_batch_vortex_gradient, vt_batch_exact_vortex).
+vt_batch_vortex_gradient, vt_batch_adjoint_vortex, vt_batch_exact_vortex).
 
 Every instance's objective J_i and gradient dJ_i/dU0_i must equal the
 single-mesh adjoint of the same instance (vt_adjoint), which
@@ -117,6 +117,29 @@ def test_batch_parameter_gradient(vt, gpu):
             Jpm.append(b.mass_l2_objective(tg)[0][k])
         fd = (Jpm[0] - Jpm[1]) / (2 * h)
         assert abs(fd - grad[k, a]) < 1e-6 * abs(grad[k, a]) + 1e-12, (a, fd, grad[k, a])
+
+
+def test_batch_parameter_gradient_on_the_device(vt, gpu):
+    # vt_batch_adjoint_vortex: the same J and the same contraction as
+    # adjoint() + vortex_gradient() on the host, with dJ/dU0 left on the
+    # device (device exp / pow and the block-tree summation order: relative
+    # 1e-11), and the state advanced to U_T as by adjoint().
+    meshes, b = _build(vt, CFG)
+    eps, x0, y0 = ([c[k] for c in CFG] for k in (3, 4, 5))
+    tg = _targets(b, DT * NSTEPS)
+    U0 = [b.state(i) for i in range(b.n)]
+    J, lam = b.adjoint(DT, NSTEPS, tg, every=EVERY)
+    UT = [b.state(i) for i in range(b.n)]
+    ref = b.vortex_gradient(0.0, eps, x0, y0, lam)
+    for i in range(b.n):
+        b.set_state(i, U0[i])
+    b.time = 0.0
+    J2, grad = b.adjoint_vortex(DT, NSTEPS, tg, eps, x0, y0, every=EVERY)
+    assert np.array_equal(J2, J)
+    for i in range(b.n):
+        assert np.array_equal(b.state(i), UT[i])
+        assert np.allclose(grad[i], ref[i], rtol=1e-11, atol=1e-13 * np.abs(ref[i]).max()), (i, grad[i], ref[i])
+    b.close()
 
 
 def test_batch_adjoint_names_the_failing_instance(vt, oracle, gpu):

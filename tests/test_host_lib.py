@@ -163,6 +163,8 @@ def test_batch_adjoint_entry_points_reject_null_handles(vt):
     assert L.vt_batch_adjoint(None, 1e-3, 4, 2, vt._d(d), vt._d(d), vt._d(d)) == vt.VT_ERR_ARG
     assert L.vt_batch_mass_l2_objective(None, vt._d(d), vt._d(d), vt._d(d)) == vt.VT_ERR_ARG
     assert L.vt_batch_vortex_gradient(None, 0.0, vt._d(d), vt._d(d), vt._d(d), vt._d(d), vt._d(d)) == vt.VT_ERR_ARG
+    assert L.vt_batch_adjoint_vortex(None, 1e-3, 4, 2, vt._d(d), vt._d(d), vt._d(d), vt._d(d), vt._d(d),
+                                     vt._d(d)) == vt.VT_ERR_ARG
     assert L.vt_batch_exact_vortex(None, 0, 0.0, 0.3, 0.0, 0.0, vt._d(d)) == vt.VT_ERR_ARG
 
 

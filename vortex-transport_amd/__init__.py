@@ -125,6 +125,8 @@ def lib():
             "vt_batch_mass_l2_objective": (ctypes.c_int, [vp, _dp, _dp, _dp]),
             "vt_batch_adjoint": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
             "vt_batch_vortex_gradient": (ctypes.c_int, [vp, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]),
+            "vt_batch_adjoint_vortex": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                                       _dp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -605,6 +607,20 @@ class Batch:
         g = np.empty(4 * rt.size)  # every entry is written
         _check(lib().vt_batch_adjoint(self._h, dt, nsteps, every, _d(rt), _d(J), _d(g)), self)
         return J, self._split(g)
+
+    def adjoint_vortex(self, dt, nsteps, rho_targets, epsilon, x0, y0, every=32):
+        """adjoint() and vortex_gradient() at the current time in one call,
+        with dJ_i/dU0_i kept on the device: (J [n], dJ_i/d(epsilon_i, x0_i,
+        y0_i) [n][3]) for type 0 vortex initial states."""
+        rt = self._targets(rho_targets)
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (self.n,)))
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(x0, dtype=np.float64), (self.n,)))
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(y0, dtype=np.float64), (self.n,)))
+        J = np.zeros(self.n)
+        grad = np.zeros((self.n, 3))
+        _check(lib().vt_batch_adjoint_vortex(self._h, dt, nsteps, every, _d(rt), _d(e), _d(a), _d(c), _d(J),
+                                             _d(grad)), self)
+        return J, grad
 
     def vortex_gradient(self, t, epsilon, x0, y0, dJdU0):
         """dJ_i/d(epsilon_i, x0_i, y0_i) [n][3] from the per-instance dJ/dU0

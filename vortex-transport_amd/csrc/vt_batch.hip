@@ -236,6 +236,80 @@ int scatter_group_gradient(vt_batch* b, int gi, const std::vector<size_t>& off, 
     return VT_OK;
 }
 
+// dJ_i/d(eps_i, x0_i, y0_i) of every member of group gi from its lam planes,
+// on the device: one launch per member's element range, the block partial
+// sums added on the host in block order.  t: the time of the initial state.
+int group_vortex_gradient(vt_batch* b, int gi, double t, const double* eps, const double* x0, const double* y0,
+                          double* grad) {
+    vt_ctx* c = b->g[gi];
+    const std::vector<int>& mem = b->members[gi];
+    std::vector<int> boff(mem.size() + 1, 0);
+    for (size_t k = 0; k < mem.size(); ++k) boff[k + 1] = boff[k] + (b->ne[mem[k]] + vt::kVGradBlock - 1) / vt::kVGradBlock;
+    const int need = 3 * boff[mem.size()];
+    if (c->part_cap < need) {
+        if (c->part) (void)hipFree(c->part);
+        c->part = nullptr;
+        c->part_cap = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->part), sizeof(double) * need));
+        c->part_cap = need;
+    }
+    vt::VGradParams prm{};
+    if (!vt::reference_nodes(c->p, prm.nodes)) return VT_ERR_UNSUPPORTED_P;
+    prm.ld = c->ld;
+    prm.t = t;
+    for (size_t k = 0; k < mem.size(); ++k) {
+        const int i = mem[k];
+        prm.lam = c->adj + b->eoff[i];
+        prm.vtx = c->vtx + b->eoff[i];
+        prm.part = c->part + 3 * boff[k];
+        prm.ne = b->ne[i];
+        prm.eps = eps[i];
+        prm.x0 = x0[i];
+        prm.y0 = y0[i];
+        HIP_TRY(vt::launch_vortex_grad(prm, c->p, c->stream));
+    }
+    double* h = vti::host_stage(c, static_cast<size_t>(need));
+    if (!h) return VT_ERR_HIP;
+    HIP_TRY(hipMemcpyAsync(h, c->part, sizeof(double) * need, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < mem.size(); ++k) {
+        double g[3] = {0.0, 0.0, 0.0};
+        for (int bl = boff[k]; bl < boff[k + 1]; ++bl)
+            for (int a = 0; a < 3; ++a) g[a] += h[3 * bl + a];
+        for (int a = 0; a < 3; ++a) grad[3 * mem[k] + a] = g[a];
+    }
+    return VT_OK;
+}
+
+// The ensemble adjoint, group by group: each group is one mesh (the disjoint
+// union of its instances), recorded, evaluated and swept by the single-mesh
+// adjoint; gather(gi) then reads the group's dJ/dU0 from its lam planes before
+// the group's recording (the bulk of device memory here) is released.
+template <class Gather>
+int batch_adjoint(vt_batch* b, double dt, int nsteps, int every, const double* rho_target, double* J, Gather gather) {
+    const std::vector<size_t> off = node_offsets(b);
+    for (int gi = 0; gi < static_cast<int>(b->g.size()); ++gi) {
+        vt_ctx* c = b->g[gi];
+        int rc = vt_adjoint_forward(c, dt, nsteps, every);
+        if (rc == VT_ERR_NONPHYSICAL) {
+            const int le = c->last_err[0];
+            int inst = b->members[gi][0];
+            for (int m : b->members[gi])
+                if (le >= b->eoff[m]) inst = m;
+            b->last_err[0] = inst;
+            b->last_err[1] = le - b->eoff[inst];
+            for (int k = 1; k < 6; ++k) b->last_err[k + 1] = c->last_err[k];
+        }
+        if (!rc) rc = group_objective(b, gi, rho_target, off, J);
+        if (!rc) rc = vti::backward_sweep(c);
+        if (!rc) rc = gather(gi, off);
+        vti::free_adjoint(c);
+        if (rc) return rc;
+    }
+    for (int s = 0; s < nsteps; ++s) b->time += dt;
+    return VT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -504,31 +578,18 @@ int vt_batch_mass_l2_objective(vt_batch* b, const double* rho_target, double* J,
 int vt_batch_adjoint(vt_batch* b, double dt, int nsteps, int every, const double* rho_target, double* J,
                      double* dJdU0) {
     if (!b || !rho_target || !J || !dJdU0 || nsteps < 1 || every < 1) return VT_ERR_ARG;
-    const std::vector<size_t> off = node_offsets(b);
-    // Group by group: each group is one mesh (the disjoint union of its
-    // instances), recorded, evaluated and swept by the single-mesh adjoint.
-    for (int gi = 0; gi < static_cast<int>(b->g.size()); ++gi) {
-        vt_ctx* c = b->g[gi];
-        int rc = vt_adjoint_forward(c, dt, nsteps, every);
-        if (rc == VT_ERR_NONPHYSICAL) {
-            const int le = c->last_err[0];
-            int inst = b->members[gi][0];
-            for (int m : b->members[gi])
-                if (le >= b->eoff[m]) inst = m;
-            b->last_err[0] = inst;
-            b->last_err[1] = le - b->eoff[inst];
-            for (int k = 1; k < 6; ++k) b->last_err[k + 1] = c->last_err[k];
-        }
-        if (!rc) rc = group_objective(b, gi, rho_target, off, J);
-        if (!rc) rc = vti::backward_sweep(c);
-        if (!rc) rc = scatter_group_gradient(b, gi, off, dJdU0);
-        // The recording (checkpoints, segment states, adjoint planes) is the
-        // bulk of device memory here; it is released group by group.
-        vti::free_adjoint(c);
-        if (rc) return rc;
-    }
-    for (int s = 0; s < nsteps; ++s) b->time += dt;
-    return VT_OK;
+    return batch_adjoint(b, dt, nsteps, every, rho_target, J, [&](int gi, const std::vector<size_t>& off) {
+        return scatter_group_gradient(b, gi, off, dJdU0);
+    });
+}
+
+int vt_batch_adjoint_vortex(vt_batch* b, double dt, int nsteps, int every, const double* rho_target,
+                            const double* eps, const double* x0, const double* y0, double* J, double* grad) {
+    if (!b || !rho_target || !eps || !x0 || !y0 || !J || !grad || nsteps < 1 || every < 1) return VT_ERR_ARG;
+    const double t0 = b->time;
+    return batch_adjoint(b, dt, nsteps, every, rho_target, J, [&](int gi, const std::vector<size_t>&) {
+        return group_vortex_gradient(b, gi, t0, eps, x0, y0, grad);
+    });
 }
 
 int vt_batch_vortex_gradient(const vt_batch* b, double t, const double* eps, const double* x0, const double* y0,

@@ -112,12 +112,29 @@ struct L2Params {
     int ne, ld;
 };
 
+// Gradient of a scalar with respect to the vortex parameters (epsilon, x0,
+// y0) of the initial state, given its gradient lam = dJ/dU0 on the device:
+// per element the closed-form tangent dU0/d(eps, x0, y0) at the nodes dotted
+// with lam; each block of kVGradBlock elements writes its three partial sums
+// to part[3 * block + a].
+constexpr int kVGradBlock = 256;
+
+struct VGradParams {
+    const double* lam;   // dJ/dU0 planes
+    const double* vtx;   // vertex planes
+    double* part;        // [blocks][3]
+    double nodes[2 * 10];  // reference nodes (xi, eta)
+    double t, eps, x0, y0;
+    int ne, ld;
+};
+
 hipError_t launch_stage(const StageParams& prm, int p, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st);
 hipError_t launch_batched(const BatchParams& bp, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st);
 hipError_t launch_objective(const ObjParams& prm, int p, hipStream_t st);
 hipError_t launch_l2(const L2Params& prm, int p, hipStream_t st);
+hipError_t launch_vortex_grad(const VGradParams& prm, int p, hipStream_t st);
 
 inline int tiles_per_xcd(int ne, int block) { return ((ne + block - 1) / block + 7) / 8; }
 inline int blocks_per_xcd(int ne) { return tiles_per_xcd(ne, kBlock); }

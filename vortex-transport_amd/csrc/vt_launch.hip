@@ -106,6 +106,70 @@ __global__ void __launch_bounds__(kL2Block) l2_kernel(L2Params prm) {
     if (threadIdx.x < 4) prm.part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x * kL2Block];
 }
 
+// dJ/d(eps, x0, y0) (see VGradParams).  The tangent of the vortex initial
+// state of Element.cpp:134-160 at each node, as the host's vortex_tangent
+// forms it; per element the sum runs over nodes and variables in order, the
+// block sum is a fixed-shape tree.
+template <int P>
+__global__ void __launch_bounds__(kVGradBlock) vortex_grad_kernel(VGradParams prm) {
+    constexpr int NP = (P + 1) * (P + 2) / 2;
+    __shared__ double red[3 * kVGradBlock];
+    const int e = blockIdx.x * kVGradBlock + static_cast<int>(threadIdx.x);
+    double ga[3] = {0.0, 0.0, 0.0};
+    if (e < prm.ne) {
+        constexpr double gamma = 1.4, M_infty = 0.5, p_infty = 20.0 / 7.0, rc = 1.0;
+        const double U_infty = 1.0 / sqrt(2.0), V_infty = 1.0 / sqrt(2.0);
+        const double A = (gamma - 1) * M_infty * M_infty / (8.0 * M_PI * M_PI);
+        const double B = (U_infty * U_infty + V_infty * V_infty) / (2.0 * M_PI * rc);
+        const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
+        const unsigned oe = static_cast<unsigned>(e) * 8u;
+        const double vx0 = plane_at(prm.vtx, 0, ld8, oe), vy0 = plane_at(prm.vtx, 1, ld8, oe);
+        const double vx1 = plane_at(prm.vtx, 2, ld8, oe), vy1 = plane_at(prm.vtx, 3, ld8, oe);
+        const double vx2 = plane_at(prm.vtx, 4, ld8, oe), vy2 = plane_at(prm.vtx, 5, ld8, oe);
+        const double eps = prm.eps;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const double xi = prm.nodes[2 * i], eta = prm.nodes[2 * i + 1], f = 1.0 - xi - eta;
+            const double dx = (vx0 * f + vx1 * xi + vx2 * eta) - prm.x0 - U_infty * prm.t;
+            const double dy = (vy0 * f + vy1 * xi + vy2 * eta) - prm.y0 - V_infty * prm.t;
+            const double f0 = 1.0 - (dx * dx + dy * dy) / (rc * rc);
+            const double ex = exp(f0), eh = exp(0.5 * f0);
+            const double f1 = 1.0 - eps * eps * A * ex;
+            const double f2 = eps * B * eh;
+            const double rho = pow(f1, 1.0 / (gamma - 1.0)), pr = p_infty * pow(f1, gamma / (gamma - 1.0));
+            const double u = U_infty - f2 * dy, v = V_infty + f2 * dx;
+            const double df0[3] = {0.0, 2.0 * dx / (rc * rc), 2.0 * dy / (rc * rc)};
+            double l[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) l[k] = plane_at(prm.lam, k * NP + i, ld8, oe);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double df1 = -(a == 0 ? 2.0 * eps : 0.0) * A * ex - eps * eps * A * ex * df0[a];
+                const double df2 = (a == 0 ? B * eh : 0.0) + f2 * 0.5 * df0[a];
+                const double drho = rho / ((gamma - 1.0) * f1) * df1;
+                const double dp = pr * gamma / ((gamma - 1.0) * f1) * df1;
+                const double ddx = a == 1 ? -1.0 : 0.0, ddy = a == 2 ? -1.0 : 0.0;
+                const double du = -df2 * dy - f2 * ddy, dv = df2 * dx + f2 * ddx;
+                ga[a] += drho * l[0];
+                ga[a] += (drho * u + rho * du) * l[1];
+                ga[a] += (drho * v + rho * dv) * l[2];
+                ga[a] += (dp / (gamma - 1.0) + 0.5 * drho * (u * u + v * v) + rho * (u * du + v * dv)) * l[3];
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) red[a * kVGradBlock + threadIdx.x] = ga[a];
+    __syncthreads();
+#pragma unroll
+    for (int w = kVGradBlock / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) red[a * kVGradBlock + threadIdx.x] += red[a * kVGradBlock + threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) prm.part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x * kVGradBlock];
+}
+
 #define VT_NQ_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
 
 // The launchers (and their kernels) are instantiated in vt_kernels_nq.hip,
@@ -154,6 +218,18 @@ hipError_t launch_l2(const L2Params& prm, int p, hipStream_t st) {
     case 1: hipLaunchKernelGGL(l2_kernel<1>, grid, block, 0, st, prm); break;
     case 2: hipLaunchKernelGGL(l2_kernel<2>, grid, block, 0, st, prm); break;
     case 3: hipLaunchKernelGGL(l2_kernel<3>, grid, block, 0, st, prm); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_vortex_grad(const VGradParams& prm, int p, hipStream_t st) {
+    const dim3 grid((prm.ne + kVGradBlock - 1) / kVGradBlock), block(kVGradBlock);
+    switch (p) {
+    case 0: hipLaunchKernelGGL(vortex_grad_kernel<0>, grid, block, 0, st, prm); break;
+    case 1: hipLaunchKernelGGL(vortex_grad_kernel<1>, grid, block, 0, st, prm); break;
+    case 2: hipLaunchKernelGGL(vortex_grad_kernel<2>, grid, block, 0, st, prm); break;
+    case 3: hipLaunchKernelGGL(vortex_grad_kernel<3>, grid, block, 0, st, prm); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
