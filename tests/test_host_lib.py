@@ -193,3 +193,14 @@ def test_views_of_one_buffer_are_passed_back_without_a_copy(vt):
     assert vt._joined([p.copy() for p in parts], flat.size) is None   # not views
     assert vt._joined(parts[::-1], flat.size) is None                 # out of order
     assert vt._joined(parts[:1], flat.size) is None                   # incomplete
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # one GPU only: --gpus 2 (or a multi-rank launch) exits with a message
+    # before touching a device
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
