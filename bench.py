@@ -140,6 +140,9 @@ def batched_figure(vt, steps, dt=1e-4):
     wp = time.perf_counter() - t0
     worst = max(float(np.linalg.norm(b.state(i) - s.state()) / np.linalg.norm(s.state())) for i, s in enumerate(solvers))
     dof = sum(2 * d["cells"] ** 2 * (d["p"] + 1) * (d["p"] + 2) // 2 * 16 for d in draws)
+    # algorithmic HBM bytes per stage launch: 1,280 B per p = 3 element (the
+    # three-array RK4 form), i.e. 128 B per node, over all instances
+    stage_bytes = sum(2 * d["cells"] ** 2 * (d["p"] + 1) * (d["p"] + 2) // 2 * 128 for d in draws)
     for s in solvers:
         s.close()
     launches = b.num_launches
@@ -149,6 +152,8 @@ def batched_figure(vt, steps, dt=1e-4):
             "elements": sum(2 * d["cells"] ** 2 for d in draws), "steps": steps,
             "ms_per_step": wb / steps * 1e3, "value": dof * steps / wb, "unit": "DOF-stage updates/s",
             "per_instance_ms_per_step": wp / steps * 1e3, "speedup_vs_per_instance": wp / wb,
+            "hbm_algorithmic_GBps": stage_bytes / (wb / steps / 4) / 1e9,
+            "hbm_frac": stage_bytes / (wb / steps / 4) / 8e12,
             "launch": cfg, "launches_per_stage": launches, "max_rel_l2_vs_per_instance": worst}
 
 
