@@ -61,6 +61,8 @@ def parse_args():
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-by-order", action="store_true",
+                    help="also time the ensemble's instances of each order p batched alone, against their plain runs")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=64)
     ap.add_argument("--adjoint-steps", type=int, default=1000)
@@ -155,6 +157,47 @@ def batched_figure(vt, steps, dt=1e-4):
             "hbm_algorithmic_GBps": stage_bytes / (wb / steps / 4) / 1e9,
             "hbm_frac": stage_bytes / (wb / steps / 4) / 8e12,
             "launch": cfg, "launches_per_stage": launches, "max_rel_l2_vs_per_instance": worst}
+
+
+def batched_by_order_figure(vt, steps, dt=1e-4, block=64):
+    """The 64-instance ensemble's instances of one order p at a time (and all
+    together), batched with 64-element tiles, against the same instances
+    stepped one plain solver after another."""
+    import torch
+    draws = vt.ensemble_draws(3, 64)
+    out = {}
+    for label, sel in (("p1", (1,)), ("p2", (2,)), ("p3", (3,)), ("all", (1, 2, 3))):
+        ds = [d for d in draws if d["p"] in sel]
+        meshes = [vt.generate_mesh(nx=d["cells"], ny=d["cells"]) for d in ds]
+        b = vt.Batch(meshes, [d["p"] for d in ds], 8)
+        b.initialize_hydrodinamics(0, [d["epsilon"] for d in ds], [d["x0"] for d in ds], [d["y0"] for d in ds])
+        b.set_launch(block)
+        b.rk4(dt, 4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.rk4(dt, steps)
+        torch.cuda.synchronize()
+        wb = time.perf_counter() - t0
+        b.close()
+        solvers = []
+        for m, d in zip(meshes, ds):
+            s = vt.Solver(m, d["p"], 8)
+            s.initialize_hydrodinamics(0, d["epsilon"], d["x0"], d["y0"])
+            s.rk4(dt, 4)
+            solvers.append(s)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in solvers:
+            s.rk4(dt, steps)
+        torch.cuda.synchronize()
+        wp = time.perf_counter() - t0
+        for s in solvers:
+            s.close()
+        ne = sum(2 * d["cells"] ** 2 for d in ds)
+        out[label] = {"instances": len(ds), "elements": ne, "batched_ms_per_step": wb / steps * 1e3,
+                      "per_instance_ms_per_step": wp / steps * 1e3, "speedup": wp / wb,
+                      "ns_per_element_stage": wb / steps / 4 / ne * 1e9}
+    return out
 
 
 def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
@@ -473,6 +516,8 @@ def main():
     s.close()
     if not a.no_batched:
         line["batched"] = batched_figure(vt, a.batched_steps)
+    if a.batched_by_order:
+        line["batched_by_order"] = batched_by_order_figure(vt, a.batched_steps)
     if not a.no_adjoint:
         line["adjoint"] = adjoint_figure(vt, a.adjoint_steps)
     if not a.no_batched_adjoint:
