@@ -61,6 +61,8 @@ def parse_args():
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--orders", action="store_true",
+                    help="also time the 512 x 512 mesh at p = 1 and 2 (ns per element-stage per order)")
     ap.add_argument("--batched-by-order", action="store_true",
                     help="also time the ensemble's instances of each order p batched alone, against their plain runs")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -271,6 +273,25 @@ def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
             "parameter_gradient": "dJ_i/d(eps_i, x0_i, y0_i) contracted on the device",
             "device_state_gb": slots * slot_bytes / 1e9, "J_sum": float(np.sum(J)),
             "grad_finite": bool(np.isfinite(grad).all()), "grad_abs_max": float(np.abs(grad).max())}
+
+
+def orders_figure(vt, mesh, dt, steps=256):
+    """The main figure's mesh at p = 1, 2 and 3: ms per RK4 step and ns per
+    element-stage, the per-order costs an ensemble adds up."""
+    import torch
+    out = {}
+    for p in (1, 2, 3):
+        s = vt.Solver(mesh, p, 8)
+        s.initialize_hydrodinamics(0)
+        s.rk4(dt, 64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.rk4(dt, steps)
+        torch.cuda.synchronize()
+        w = time.perf_counter() - t0
+        out["p%d" % p] = {"ms_per_step": w / steps * 1e3, "ns_per_element_stage": w / steps / 4 / s.ne * 1e9}
+        s.close()
+    return out
 
 
 def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
@@ -516,6 +537,8 @@ def main():
     s.close()
     if not a.no_batched:
         line["batched"] = batched_figure(vt, a.batched_steps)
+    if a.orders:
+        line["orders"] = orders_figure(vt, mesh, a.dt)
     if a.batched_by_order:
         line["batched_by_order"] = batched_by_order_figure(vt, a.batched_steps)
     if not a.no_adjoint:
