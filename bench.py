@@ -117,7 +117,8 @@ def batched_figure(vt, steps, dt=1e-4):
     """Ensemble of 64 instances (seed 3: cells per side 32..256, p 1..3,
     epsilon 0.1..0.4, centre in [-2.5, 2.5]^2 of a 10 x 10 domain), one batched
     launch per RK stage with the autotuned launch shape, against the same 64
-    instances stepped one plain solver after another."""
+    instances stepped one plain solver after another; medians of three runs
+    of each."""
     import torch
     draws = vt.ensemble_draws(3, 64)
     meshes = [vt.generate_mesh(nx=d["cells"], ny=d["cells"]) for d in draws]
@@ -127,21 +128,26 @@ def batched_figure(vt, steps, dt=1e-4):
     b.initialize_hydrodinamics(0, *args)
     cfg, _ = b.autotune(dt, 8)
     b.rk4(dt, 4)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    b.rk4(dt, steps)
-    wb = time.perf_counter() - t0
     solvers = []
     for m, d in zip(meshes, draws):
         s = vt.Solver(m, d["p"], 8)
         s.initialize_hydrodinamics(0, d["epsilon"], d["x0"], d["y0"])
         s.rk4(dt, 4)
         solvers.append(s)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in solvers:
-        s.rk4(dt, steps)
-    wp = time.perf_counter() - t0
+    # three alternating runs of each (every instance advances by the same steps
+    # in both forms), medians reported
+    wbs, wps = [], []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.rk4(dt, steps)
+        wbs.append(time.perf_counter() - t0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in solvers:
+            s.rk4(dt, steps)
+        wps.append(time.perf_counter() - t0)
+    wb, wp = float(np.median(wbs)), float(np.median(wps))
     worst = max(float(np.linalg.norm(b.state(i) - s.state()) / np.linalg.norm(s.state())) for i, s in enumerate(solvers))
     dof = sum(2 * d["cells"] ** 2 * (d["p"] + 1) * (d["p"] + 2) // 2 * 16 for d in draws)
     # algorithmic HBM bytes per stage launch: 1,280 B per p = 3 element (the
@@ -156,6 +162,7 @@ def batched_figure(vt, steps, dt=1e-4):
             "elements": sum(2 * d["cells"] ** 2 for d in draws), "steps": steps,
             "ms_per_step": wb / steps * 1e3, "value": dof * steps / wb, "unit": "DOF-stage updates/s",
             "per_instance_ms_per_step": wp / steps * 1e3, "speedup_vs_per_instance": wp / wb,
+            "runs_ms_per_step": {"batched": [w / steps * 1e3 for w in wbs], "per_instance": [w / steps * 1e3 for w in wps]},
             "hbm_algorithmic_GBps": stage_bytes / (wb / steps / 4) / 1e9,
             "hbm_frac": stage_bytes / (wb / steps / 4) / 8e12,
             "launch": cfg, "launches_per_stage": launches, "max_rel_l2_vs_per_instance": worst}
@@ -206,7 +213,7 @@ def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
     """Forward + discrete adjoint (checkpoints every 32 steps) of the
     mass-weighted L2 misfit to the advected target vortex (eps 0.32, centre
     (0.1, -0.05)) on n x n cells of a 10 x 10 domain, p = 3, against the plain
-    forward solve of the same steps."""
+    forward solve of the same steps; medians of three runs of each."""
     import torch
     mesh = vt.generate_mesh(nx=n, ny=n)
     s = vt.Solver(mesh, 3, 8)
@@ -214,20 +221,25 @@ def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
     s.initialize_hydrodinamics(0)
     U0 = s.state()
     s.rk4(dt, 64)
-    s.set_state(U0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s.rk4(dt, steps)
-    tf = time.perf_counter() - t0
-    s.set_state(U0)
-    t0 = time.perf_counter()
-    J, lam0 = s.adjoint(dt, steps, tgt, every)
-    ta = time.perf_counter() - t0
+    # three alternating runs of each, medians reported
+    tfs, tas = [], []
+    for _ in range(3):
+        s.set_state(U0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.rk4(dt, steps)
+        tfs.append(time.perf_counter() - t0)
+        s.set_state(U0)
+        t0 = time.perf_counter()
+        J, lam0 = s.adjoint(dt, steps, tgt, every)
+        tas.append(time.perf_counter() - t0)
     s.close()
+    tf, ta = float(np.median(tfs)), float(np.median(tas))
     return {"workload": "%dx%d cells, 10x10 domain, p=3, integration_order 8, %d RK4 steps of dt %g, "
                         "checkpoints every %d steps, J = mass-weighted L2 misfit to the advected vortex "
                         "(eps 0.32, centre (0.1, -0.05))" % (n, n, steps, dt, every),
-            "elements": 2 * n * n, "forward_s": tf, "forward_plus_adjoint_s": ta, "ratio": ta / tf, "J": J,
+            "elements": 2 * n * n, "forward_s": tf, "forward_plus_adjoint_s": ta, "ratio": ta / tf,
+            "runs": {"forward_s": tfs, "forward_plus_adjoint_s": tas}, "J": J,
             "grad_norm": float(np.linalg.norm(lam0))}
 
 
@@ -482,6 +494,7 @@ def main():
         return
 
     s.rk4(a.dt, a.warmup)
+    s.prepare(1, a.dt, a.steps)   # the timed steps' graphs are built outside the timed region
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     s.rk4(a.dt, a.steps)          # graph replays on the library stream; returns after it drains

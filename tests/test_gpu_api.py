@@ -50,3 +50,22 @@ def test_batch_step_timed_equals_step(vt, gpu):
     for i in range(3):
         assert np.array_equal(bs[0].state(i), bs[1].state(i))
     assert bs[0].time == bs[1].time
+
+
+def test_prepare_steps_builds_graphs_without_stepping(vt, gpu):
+    mesh = vt.generate_mesh(nx=6, ny=6)
+    a = vt.Solver(mesh, 3, 8)
+    a.initialize_hydrodinamics(0)
+    b = vt.Solver(mesh, 3, 8)
+    b.initialize_hydrodinamics(0)
+    U0, t0 = b.state(), b.time
+    b.prepare(1, 1e-3, 75)
+    b.prepare(0, 1e-3, 7)
+    assert np.array_equal(b.state(), U0) and b.time == t0
+    a.rk4(1e-3, 75)
+    b.rk4(1e-3, 75)
+    assert np.array_equal(a.state(), b.state())
+    with pytest.raises(vt.VortexError):
+        b.prepare(2, 1e-3, 4)
+    a.close()
+    b.close()

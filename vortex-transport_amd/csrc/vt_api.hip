@@ -476,6 +476,19 @@ int vt_step(vt_ctx* c, int method, double dt, int nsteps) {
     return vti::run_steps(c, method, dt, nsteps, nullptr, nullptr);
 }
 
+int vt_prepare_steps(vt_ctx* c, int method, double dt, int nsteps) {
+    if (!c) return VT_ERR_ARG;
+    if (method != 0 && method != 1) return VT_ERR_METHOD;
+    if (nsteps < 0) return VT_ERR_ARG;
+    for (int k : vti::replay_plan(method, nsteps)) {
+        if (method == 0 && k == 1) continue;  // a single Euler step is launched directly
+        hipGraphExec_t g;
+        const int rc = get_graph(c, method, dt, k, &g);
+        if (rc) return rc;
+    }
+    return VT_OK;
+}
+
 int vt_step_timed(vt_ctx* c, int method, double dt, int nsteps, double* ms) {
     if (!c || !ms) return VT_ERR_ARG;
     hipEvent_t e0, e1;
