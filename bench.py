@@ -171,7 +171,7 @@ def batched_figure(vt, steps, dt=1e-4):
 def batched_by_order_figure(vt, steps, dt=1e-4, block=64):
     """The 64-instance ensemble's instances of one order p at a time (and all
     together), batched with 64-element tiles, against the same instances
-    stepped one plain solver after another."""
+    stepped one plain solver after another; medians of three runs of each."""
     import torch
     draws = vt.ensemble_draws(3, 64)
     out = {}
@@ -182,24 +182,27 @@ def batched_by_order_figure(vt, steps, dt=1e-4, block=64):
         b.initialize_hydrodinamics(0, [d["epsilon"] for d in ds], [d["x0"] for d in ds], [d["y0"] for d in ds])
         b.set_launch(block)
         b.rk4(dt, 4)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        b.rk4(dt, steps)
-        torch.cuda.synchronize()
-        wb = time.perf_counter() - t0
-        b.close()
         solvers = []
         for m, d in zip(meshes, ds):
             s = vt.Solver(m, d["p"], 8)
             s.initialize_hydrodinamics(0, d["epsilon"], d["x0"], d["y0"])
             s.rk4(dt, 4)
             solvers.append(s)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for s in solvers:
-            s.rk4(dt, steps)
-        torch.cuda.synchronize()
-        wp = time.perf_counter() - t0
+        # three alternating runs of each, medians (the first builds the graphs)
+        wbs, wps = [], []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.rk4(dt, steps)
+            torch.cuda.synchronize()
+            wbs.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            for s in solvers:
+                s.rk4(dt, steps)
+            torch.cuda.synchronize()
+            wps.append(time.perf_counter() - t0)
+        wb, wp = float(np.median(wbs)), float(np.median(wps))
+        b.close()
         for s in solvers:
             s.close()
         ne = sum(2 * d["cells"] ** 2 for d in ds)
