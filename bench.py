@@ -264,6 +264,7 @@ def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
     U0 = [b.state(i) for i in range(b.n)]
     tg = [b.exact_vortex(i, dt * steps, 0.32, 0.1, -0.05)[:, :, 0] for i in range(b.n)]
     b.rk4(dt, 4)
+    b.prepare(1, dt, steps)
     for i in range(b.n):
         b.set_state(i, U0[i])
     b.time = 0.0

@@ -209,6 +209,8 @@ int vt_batch_set_time(vt_batch *b, double t);
 /* vt_step over all instances; VT_ERR_NONPHYSICAL at the first failing flux
  * evaluation of any instance, located by vt_batch_last_error. */
 int vt_batch_step(vt_batch *b, int method, double dt, int nsteps);
+/* vt_prepare_steps for an ensemble (no reference counterpart). */
+int vt_batch_prepare_steps(vt_batch *b, int method, double dt, int nsteps);
 int vt_batch_step_timed(vt_batch *b, int method, double dt, int nsteps, double *ms);
 /* instance, element (local), quadrature point, side, code, step, stage. */
 int vt_batch_last_error(const vt_batch *b, int *info /* [7] */);

@@ -244,3 +244,16 @@ def test_ensemble_64_autotuned_equals_every_instance(vt, gpu):
         U = _single(vt, m, (d["cells"], d["p"], 8, d["epsilon"], d["x0"], d["y0"]), 1, 1e-4, 6)
         assert same(b.state(i), U), i
     b.close()
+
+
+def test_batch_prepare_builds_graphs_without_stepping(vt, gpu):
+    meshes, b = _build(vt, SMALL)
+    U0 = [b.state(i) for i in range(b.n)]
+    b.prepare(1, 1e-3, 70)
+    assert b.time == 0.0
+    for i in range(b.n):
+        assert np.array_equal(b.state(i), U0[i])
+    b.rk4(1e-3, 70)
+    for i, (m, c) in enumerate(zip(meshes, SMALL)):
+        assert same(b.state(i), _single(vt, m, c, 1, 1e-3, 70)), i
+    b.close()

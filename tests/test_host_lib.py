@@ -198,6 +198,7 @@ def test_views_of_one_buffer_are_passed_back_without_a_copy(vt):
 def test_prepare_steps_rejects_bad_arguments(vt):
     L = vt.lib()
     assert L.vt_prepare_steps(None, 1, 1e-3, 10) == vt.VT_ERR_ARG
+    assert L.vt_batch_prepare_steps(None, 1, 1e-3, 10) == vt.VT_ERR_ARG
 
 
 def test_bench_refuses_more_than_one_gpu():

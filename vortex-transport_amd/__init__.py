@@ -93,6 +93,7 @@ def lib():
             "vt_rhs": (ctypes.c_int, [vp, _dp]),
             "vt_step": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
             "vt_prepare_steps": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+            "vt_batch_prepare_steps": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
             "vt_last_error": (ctypes.c_int, [vp, _ip]),
             "vt_step_timed": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
             "vt_stage_kernel_times": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, _dp]),
@@ -548,6 +549,11 @@ class Batch:
 
     def rk4(self, dt, nsteps=1):
         self.step(1, dt, nsteps)
+
+    def prepare(self, method, dt, nsteps):
+        """Builds the HIP graphs step(method, dt, nsteps) replays, without
+        running them."""
+        _check(lib().vt_batch_prepare_steps(self._h, method, dt, nsteps), self)
 
     def forward_euler(self, dt, nsteps=1):
         self.step(0, dt, nsteps)

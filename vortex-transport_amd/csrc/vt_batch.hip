@@ -463,6 +463,19 @@ int vt_batch_step(vt_batch* b, int method, double dt, int nsteps) {
     return run_steps(b, method, dt, nsteps, nullptr, nullptr);
 }
 
+int vt_batch_prepare_steps(vt_batch* b, int method, double dt, int nsteps) {
+    if (!b) return VT_ERR_ARG;
+    if (method != 0 && method != 1) return VT_ERR_METHOD;
+    if (nsteps < 0) return VT_ERR_ARG;
+    for (int k : vti::replay_plan(method, nsteps)) {
+        if (method == 0 && k == 1) continue;  // a single Euler step is launched directly
+        hipGraphExec_t g;
+        const int rc = get_graph(b, method, dt, k, &g);
+        if (rc) return rc;
+    }
+    return VT_OK;
+}
+
 int vt_batch_step_timed(vt_batch* b, int method, double dt, int nsteps, double* ms) {
     if (!b || !ms) return VT_ERR_ARG;
     hipEvent_t e0, e1;
