@@ -74,6 +74,24 @@ std::vector<int> replay_plan(int method, int nsteps) {
     return reps;
 }
 
+int finish_capture(hipStream_t st, hipError_t launch_err, hipGraphExec_t* ex) {
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(st, &g);
+    const bool ok = hip_ok(launch_err, "graph capture launch") && hip_ok(e2, "hipStreamEndCapture");
+    if (!ok) {
+        if (e2 == hipSuccess && g) (void)hipGraphDestroy(g);
+        return VT_ERR_HIP;
+    }
+    const hipError_t e3 = hipGraphInstantiate(ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (!hip_ok(e3, "hipGraphInstantiate")) return VT_ERR_HIP;
+    if (!hip_ok(hipGraphUpload(*ex, st), "hipGraphUpload") || !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) {
+        (void)hipGraphExecDestroy(*ex);
+        return VT_ERR_HIP;
+    }
+    return VT_OK;
+}
+
 int ensure_err_slots(unsigned long long** slots, size_t* cap, size_t n) {
     if (n <= *cap) return VT_OK;
     if (*slots) hipHostFree(*slots);
@@ -145,19 +163,15 @@ int get_graph(vt_ctx* c, int method, double dt, int k, hipGraphExec_t* out) {
     auto key = std::make_tuple(method, dbits, k);
     auto it = c->graphs.find(key);
     if (it != c->graphs.end()) { *out = it->second; return VT_OK; }
-    hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int i = 0; i < k && e == hipSuccess; ++i) {
         if (method == 1) e = enqueue_rk4_step(c, dt, 4 * i);
         else e = enqueue_euler_step(c, dt, (i % 2) ? c->Ua : c->Un, (i % 2) ? c->Un : c->Ua, i);
     }
-    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-    if (!hip_ok(e, "graph capture launch") || !hip_ok(e2, "hipStreamEndCapture")) return VT_ERR_HIP;
     hipGraphExec_t ex;
-    hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    if (!hip_ok(e3, "hipGraphInstantiate")) return VT_ERR_HIP;
+    const int rc = vti::finish_capture(c->stream, e, &ex);
+    if (rc) return rc;
     c->graphs[key] = ex;
     *out = ex;
     return VT_OK;

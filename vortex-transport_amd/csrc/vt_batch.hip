@@ -117,19 +117,15 @@ int get_graph(vt_batch* b, int method, double dt, int k, hipGraphExec_t* out) {
     auto key = std::make_tuple(method, dbits, k);
     auto it = b->graphs.find(key);
     if (it != b->graphs.end()) { *out = it->second; return VT_OK; }
-    hipGraph_t gr;
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int i = 0; i < k && e == hipSuccess; ++i) {
         if (method == 1) e = enqueue_rk4_step(b, dt, 4 * i);
         else e = enqueue_euler_step(b, dt, (i % 2) != 0, i);
     }
-    hipError_t e2 = hipStreamEndCapture(b->stream, &gr);
-    if (!hip_ok(e, "graph capture launch") || !hip_ok(e2, "hipStreamEndCapture")) return VT_ERR_HIP;
     hipGraphExec_t ex;
-    hipError_t e3 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-    hipGraphDestroy(gr);
-    if (!hip_ok(e3, "hipGraphInstantiate")) return VT_ERR_HIP;
+    const int rc = vti::finish_capture(b->stream, e, &ex);
+    if (rc) return rc;
     b->graphs[key] = ex;
     *out = ex;
     return VT_OK;

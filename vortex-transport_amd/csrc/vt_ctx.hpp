@@ -109,6 +109,12 @@ int backward_sweep(vt_ctx* c);
 // Pinned host error slots, at least n of them.
 int ensure_err_slots(unsigned long long** slots, size_t* cap, size_t n);
 
+// Ends a stream capture whose launches returned launch_err, instantiates the
+// captured graph into *ex and uploads it to the device on the stream (the
+// first replay of a graph otherwise pays for that upload), waiting for the
+// upload.  The captured graph is destroyed in every case.
+int finish_capture(hipStream_t st, hipError_t launch_err, hipGraphExec_t* ex);
+
 }  // namespace vti
 
 #define HIP_TRY(expr)                                        \
