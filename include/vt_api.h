@@ -168,6 +168,13 @@ int vt_rhs_vjp(vt_ctx *ctx, const double *w, double *g);
  * vt_step, keeping the state at the start of every `every` steps on the
  * device (ceil(nsteps/every) states).  The state ends at U_T. */
 int vt_adjoint_forward(vt_ctx *ctx, double dt, int nsteps, int every);
+/* Identifies the recording vt_adjoint_backward would sweep: *id = the
+ * number of vt_adjoint_forward calls (vt_adjoint makes one) made on the
+ * context so far when the last of them completed, 0 when there is no
+ * complete recording.  A caller that records several runs on one context
+ * keeps the id of each and re-records a run whose id is no longer current
+ * before sweeping it. */
+int vt_adjoint_recording(const vt_ctx *ctx, unsigned long long *id);
 /* Reverse sweep over the last recorded forward run: lam0 = (dU_T/dU_0)^T lamT.
  * Stage states are recomputed one segment (`every` steps) at a time from the
  * checkpoints; 4*every + 1 states of device memory.  The current state is

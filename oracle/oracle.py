@@ -312,7 +312,7 @@ def write_input(path, p=3, nx=8, ny=8, domain_x="10.0", domain_y="10.0", perturb
 
 
 def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, threads=None, no_state=False,
-                  env_extra=None):
+                  env_extra=None, write_dumps=False):
     """Runs oracle/_ref/vt_ref_run (the reference's own TUs) in workdir; returns a dict of arrays."""
     if not os.path.exists(REF_RUN):
         raise FileNotFoundError(REF_RUN)
@@ -332,6 +332,8 @@ def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, thr
         cmd.append("--rhs-at-end")
     if no_state:
         cmd.append("--no-state")
+    if write_dumps:
+        cmd.append("--write-dumps")
     if mesh is not None:
         v, b, t = mesh
         ne = v.shape[0]

@@ -17,7 +17,11 @@
 //
 // Usage:
 //   vt_ref_run run  <input> <out.bin> --minv F --stiff F [--mesh F] [--steps N]
-//                   [--rhs-at-end] [--no-state]
+//                   [--rhs-at-end] [--no-state] [--write-dumps]
+// --write-dumps writes the reference's text output into the current directory
+// as its driver does (main.cpp:119-121,136,171-177): output/step_0 after the
+// set-up and output/step_N every write_every_steps steps (write_output,
+// Utilities.cpp:65-76, Element::write_data, Element.cpp:336-425).
 //   vt_ref_run flux <in.bin> <out.bin>      records of UL[4] UR[4] n[2] -> flux[4]
 #include <chrono>
 #include <cstdio>
@@ -33,6 +37,7 @@
 #include "Parameters.H"
 #include "Quadraturerule.H"
 #include "Timestepping.H"
+#include "Utilities.H"
 
 // Defined in Timestepping.cpp:18 with external linkage but declared in no header.
 void evolve_elem(Evolve_element* evo_elemts, int n_elements);
@@ -73,7 +78,7 @@ int main(int argc, char* argv[]) {
     }
     const char *minv_path = 0, *stiff_path = 0, *mesh_path = 0;
     int steps = -1;
-    bool rhs_at_end = false, no_state = false;
+    bool rhs_at_end = false, no_state = false, write_dumps = false;
     for (int i = 4; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--minv" && i + 1 < argc) minv_path = argv[++i];
@@ -82,6 +87,7 @@ int main(int argc, char* argv[]) {
         else if (a == "--steps" && i + 1 < argc) steps = std::atoi(argv[++i]);
         else if (a == "--rhs-at-end") rhs_at_end = true;
         else if (a == "--no-state") no_state = true;
+        else if (a == "--write-dumps") write_dumps = true;
         else { std::fprintf(stderr, "unknown option %s\n", argv[i]); return 2; }
     }
     if (!minv_path || !stiff_path) { std::fprintf(stderr, "--minv and --stiff are required\n"); return 2; }
@@ -136,6 +142,11 @@ int main(int argc, char* argv[]) {
         elements[i].build_stiffness_matrix(stiffness_matrix);
         elements[i].initialize_hydrodinamics(parms.U_initialization_type, gauss_integral_area);
     }
+    if (write_dumps) {
+        clean_create_directory("output");
+        clean_create_directory("output/step_0");
+        write_output(elements, ne, 0);
+    }
     Evolve_element* evolve_elements = new Evolve_element[ne];
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < ne; ++i) {
@@ -185,6 +196,10 @@ int main(int argc, char* argv[]) {
         if (parms.stepping_method == 0) forward_euler(elements, evolve_elements, parms.time_step, ne);
         else if (parms.stepping_method == 1) rk4(elements, evolve_elements, parms.time_step, ne);
         else { std::printf("ERROR: Unsupported time stepping method\n"); return 1; }
+        if (write_dumps && a % parms.write_every_steps == 0) {
+            clean_create_directory("output/step_" + std::to_string(a));
+            write_output(elements, ne, a);
+        }
     }
     auto t1 = std::chrono::steady_clock::now();
     double sec = std::chrono::duration<double>(t1 - t0).count();

@@ -2,9 +2,12 @@
 // reference driver does (main.cpp:96-131) through the drop-in API and dumps
 // per element: type, right, left, vertical, vertices[6], jacobian[4], detJ,
 // invJ[4], detInvJ, normals[6], lengths[3], nodes[2 Np], then U0 and F0, as
-// raw doubles.  No GPU call is made.
-//   dropin_check <input_file> <out.bin>
+// raw doubles.  With --write it also writes the step-0 text output into the
+// current directory as the reference driver does (main.cpp:119-121,136).
+// No GPU call is made.
+//   dropin_check <input_file> <out.bin> [--write]
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "Element.H"
@@ -12,6 +15,7 @@
 #include "Parameters.H"
 #include "Preevolve.H"
 #include "Quadraturerule.H"
+#include "Utilities.H"
 
 int main(int argc, char* argv[]) {
     char* pargv[2] = {argv[0], argv[1]};
@@ -29,6 +33,11 @@ int main(int argc, char* argv[]) {
         el[i].build_mass_matrix_inverse(minv);
         el[i].build_stiffness_matrix(stiff);
         el[i].initialize_hydrodinamics(prm.U_initialization_type, area);
+    }
+    if (argc > 3 && std::strcmp(argv[3], "--write") == 0) {
+        clean_create_directory("output");
+        clean_create_directory("output/step_0");
+        write_output(el.data(), n, 0);
     }
     FILE* f = std::fopen(argv[2], "wb");
     auto put = [&](double x) { std::fwrite(&x, sizeof(double), 1, f); };

@@ -101,10 +101,36 @@ def make_flux(name="roe_flux_known_answers"):
     print(name, "written", len(recs))
 
 
+# The reference's text output (grid/ and output/step_*/ files) of a small run,
+# as its own writers produce it: input keys, steps.
+TEXT_CASE = ("text_dumps_p3_o8_4x4_perturb1",
+             dict(p=3, nx=4, ny=4, integration_order=8, perturb_grid=1, simulation_time="0.028284271247",
+                  number_time_steps=20, write_every_steps=10), 20)
+
+
+def make_text_dumps():
+    import json
+    name, kw, steps = TEXT_CASE
+    files = {}
+    with tempfile.TemporaryDirectory() as d:
+        O.run_reference(d, kw, steps, write_dumps=True)
+        for sub in ("grid", "output"):
+            for dirpath, _, names in os.walk(os.path.join(d, sub)):
+                for f in names:
+                    path = os.path.join(dirpath, f)
+                    with open(path, "rb") as fh:
+                        files[os.path.relpath(path, d)] = fh.read().decode("ascii")
+    with open(os.path.join(OUT, name + ".json"), "w") as fh:
+        json.dump({"input": kw, "steps": steps, "files": dict(sorted(files.items()))}, fh, indent=0)
+    print(name, "written", len(files), "files")
+
+
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(CASES) + ["roe_flux_known_answers"]
+    names = sys.argv[1:] or list(CASES) + ["roe_flux_known_answers", TEXT_CASE[0]]
     for n in names:
-        if n == "roe_flux_known_answers":
+        if n == TEXT_CASE[0]:
+            make_text_dumps()
+        elif n == "roe_flux_known_answers":
             make_flux()
         else:
             make(n, *CASES[n])

@@ -68,6 +68,41 @@ def test_dropin_host_setup_bit_exact(check_exe, tmp_path, name):
     assert len(glob.glob(str(tmp_path / "grid" / "element_*.txt"))) == ne
 
 
+TEXT_FIXTURE = os.path.join(GOLD, "text_dumps_p3_o8_4x4_perturb1.json")
+
+
+def _text_fixture(tmp):
+    import json
+    with open(TEXT_FIXTURE) as f:
+        d = json.load(f)
+    return _input(tmp, **d["input"]), d["files"]
+
+
+def _tree(workdir):
+    out = {}
+    for sub in ("grid", "output"):
+        for dirpath, _, names in os.walk(os.path.join(workdir, sub)):
+            for f in names:
+                path = os.path.join(dirpath, f)
+                with open(path, "rb") as fh:
+                    out[os.path.relpath(path, workdir)] = fh.read().decode("ascii")
+    return out
+
+
+def test_dropin_step0_text_output_byte_identical(check_exe, tmp_path):
+    """grid/element_*.txt (Meshgeneration.cpp:160-179) and the step-0 dumps
+    output/step_0/element_*.txt and JMS_element_*.txt (Element.cpp:336-425)
+    written through the drop-in equal the reference's own files byte for
+    byte (a 4 x 4 perturbed mesh, p = 3)."""
+    inp, want = _text_fixture(str(tmp_path))
+    subprocess.run([check_exe, inp, str(tmp_path / "out.bin"), "--write"], cwd=str(tmp_path), check=True)
+    got = _tree(str(tmp_path))
+    step0 = {k: v for k, v in want.items() if not k.startswith("output/step_") or k.startswith("output/step_0/")}
+    assert sorted(got) == sorted(step0)
+    bad = [k for k in step0 if got[k] != step0[k]]
+    assert not bad, bad[:5]
+
+
 def _read_output(workdir, step, ne):
     vals = []
     for e in range(ne):
@@ -97,3 +132,22 @@ def test_drivers_reproduce_reference_output(gpu, tmp_path, exe):
         assert np.abs(out[:, :, 4:8] - want).max() <= 5.01e-7, step
         if d is not None:
             assert np.allclose(out[:, :, 1], float("%f" % d["time"]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("exe", ["vortex_transport_gpu", "vt_main_on_gpu"])
+def test_drivers_text_output_byte_identical(gpu, tmp_path, exe):
+    """Both drivers (the drop-in's and the reference's own main.cpp built on
+    the drop-in) write grid/ and output/step_{0,10,20}/ files equal to the
+    reference's byte for byte: every value printed with std::to_string's six
+    decimals, from states that agree with the reference's to ~1e-15."""
+    path = os.path.join(HOST, exe) if exe == "vortex_transport_gpu" else os.path.join(ROOT, "oracle", "_ref", exe)
+    if not os.path.exists(path):
+        pytest.skip("%s not built" % exe)
+    inp, want = _text_fixture(str(tmp_path))
+    r = subprocess.run([path, inp], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = _tree(str(tmp_path))
+    assert sorted(got) == sorted(want)
+    bad = [k for k in want if got[k] != want[k]]
+    assert not bad, bad[:5]

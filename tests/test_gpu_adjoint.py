@@ -165,6 +165,36 @@ def test_torch_autograd_pipeline(vt, gpu):
     assert abs((Jp - Jm) / (2 * h) - params.grad[0].item()) < 1e-6 * abs(params.grad[0].item())
 
 
+def test_torch_two_runs_on_one_solver(vt, gpu):
+    """Two rk4_solve nodes on one solver in one graph, with an unrelated
+    adjoint call recorded between the forward and the backward pass: each
+    node sweeps its own run (re-recorded when replaced), and the gradient
+    equals the C ABI's adjoint over the combined run."""
+    import torch
+    import vortex_transport_amd.torch_ops as T
+    dt, n1, n2 = 1e-3, 12, 9
+    mesh = vt.generate_mesh(nx=8, ny=8, perturb_grid=1.0)
+    s = vt.Solver(mesh, 3, 8)
+    tgt = s.exact_vortex(dt * (n1 + n2), 0.32, 0.1, -0.05)[:, :, 0]
+    s.initialize_hydrodinamics(0, 0.3, 0.2, -0.1)
+    u0 = s.state()
+    U0 = torch.from_numpy(u0.copy()).requires_grad_(True)
+    U1 = T.rk4_solve(U0, s, dt, n1, every=4)
+    U2 = T.rk4_solve(U1, s, dt, n2, every=4)
+    J = T.mass_l2(U2, s, tgt)
+    # an unrelated recording on the same solver before the backward pass
+    s.set_state(u0)
+    s.adjoint(dt, 5, tgt, every=2)
+    before, t_before = s.state(), s.time
+    J.backward()
+    assert np.array_equal(s.state(), before) and s.time == t_before
+    ref = vt.Solver(mesh, 3, 8)
+    ref.set_state(u0)
+    Jr, gr = ref.adjoint(dt, n1 + n2, tgt, every=4)
+    assert abs(J.item() - Jr) <= 1e-13 * abs(Jr)
+    assert rel(U0.grad.numpy(), gr) < 1e-13
+
+
 # --- the adjoint workload: 256 x 256 cells, p = 3, 1000 RK4 steps ------------
 # dt = 1e-4: at h = 10/256 the reference numerics leave the physical range in
 # step 4 with dt = 1e-3 (test_nonphysical_state_reported_like_the_reference).

@@ -329,6 +329,25 @@ def test_l2_state_error_matches_oracle(vt, oracle, gpu, p):
         assert np.all(np.abs(err - ref) <= 1e-12 * ref), (err, ref)
 
 
+def test_l2_state_error_cubic_closed_form(vt, gpu):
+    """The device metric against the closed form for a cubic nodal difference
+    (exact rational basis integrals), which square-then-interpolate and
+    interpolate-then-square give differently; tolerance 1e-10 relative for
+    the difference's rounding in U_ref + d - U_ref (|d| ~ 1e-3, |U| ~ 8)."""
+    import l2_closed_form as L
+    mesh = vt.generate_mesh(nx=6, ny=5, perturb_grid=1.0)
+    s = vt.Solver(mesh, 3, 8)
+    s.initialize_hydrodinamics(0)
+    Uref = s.state()
+    d = L.cubic_difference(mesh.element_coordinates)
+    want, other = L.metric(mesh.element_coordinates, d)
+    assert np.all(np.abs(want - other) > 1e-4 * want)  # ~1e-3 apart: 1e7 times the tolerance
+    s.set_state(Uref + d)
+    got = s.l2_state_error(Uref)
+    assert np.allclose(got, want, rtol=1e-10, atol=0), (got, want)
+    s.close()
+
+
 # --- element numbering without the cell pairs -----------------------------------
 
 def _relabel(vt, mesh, shift):

@@ -314,6 +314,24 @@ def test_l2_metric_closed_form(oracle):
     assert np.allclose(got, 2 * delta, rtol=1e-12)
 
 
+def test_l2_metric_cubic_closed_form(oracle):
+    """A cubic nodal difference separates the script's square-then-interpolate
+    formula from interpolate-then-square; the oracle must give the former."""
+    import l2_closed_form as L
+    from fractions import Fraction
+    c = L.basis_integrals()
+    assert [c[j] for j in (0, 3, 9)] == [Fraction(1, 60)] * 3 and c[5] == Fraction(9, 40)
+    assert all(c[j] == Fraction(3, 80) for j in (1, 2, 4, 6, 7, 8)) and sum(c) == Fraction(1, 2)
+    v, b, t = oracle.generate_mesh(6, 5, 10.0, 10.0, 1.0)
+    d = L.cubic_difference(v)
+    want, other = L.metric(v, d)
+    # the two formulas differ by ~1e-3 relative here, 1e7 times the tolerance below
+    assert np.all(np.abs(want - other) > 1e-4 * want), (want, other)
+    base = np.zeros_like(d)
+    got = oracle.l2_state_error(v, base, d)
+    assert np.allclose(got, want, rtol=1e-13, atol=0), (got, want)
+
+
 # --- against the reference's own translation units (golden fixtures) --------------
 
 def _fixtures():

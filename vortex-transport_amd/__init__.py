@@ -101,6 +101,7 @@ def lib():
             "vt_rhs_vjp": (ctypes.c_int, [vp, _dp, _dp]),
             "vt_adjoint_forward": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
             "vt_adjoint_backward": (ctypes.c_int, [vp, _dp, _dp]),
+            "vt_adjoint_recording": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_ulonglong)]),
             "vt_vortex_tangent": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
             "vt_mass_l2_objective": (ctypes.c_int, [vp, _dp, _dp, _dp]),
             "vt_adjoint": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
@@ -418,6 +419,13 @@ class Solver:
     def adjoint_forward(self, dt, nsteps, every=32):
         """RK4 steps as rk4(), recording a checkpoint every `every` steps."""
         _check(lib().vt_adjoint_forward(self._h, dt, nsteps, every), self)
+
+    def adjoint_recording(self):
+        """Id of the recording adjoint_backward sweeps (0: none); it changes
+        with every adjoint_forward / adjoint call."""
+        r = ctypes.c_ulonglong()
+        _check(lib().vt_adjoint_recording(self._h, ctypes.byref(r)))
+        return r.value
 
     def adjoint_backward(self, lamT):
         """(dU_T/dU_0)^T lamT over the last recorded forward run."""

@@ -225,6 +225,7 @@ int vt_rhs_vjp(vt_ctx* c, const double* w, double* g) {
 int vt_adjoint_forward(vt_ctx* c, double dt, int nsteps, int every) {
     if (!c || nsteps < 1 || every < 1) return VT_ERR_ARG;
     c->recorded = false;
+    ++c->rec_id;
     const int nseg = (nsteps + every - 1) / every;
     const size_t n = slot_elems(c);
     if (static_cast<int>(c->ckpt.size()) < nseg) {
@@ -247,6 +248,12 @@ int vt_adjoint_forward(vt_ctx* c, double dt, int nsteps, int every) {
     c->rec_every = every;
     c->rec_dt = dt;
     c->recorded = true;
+    return VT_OK;
+}
+
+int vt_adjoint_recording(const vt_ctx* c, unsigned long long* id) {
+    if (!c || !id) return VT_ERR_ARG;
+    *id = c->recorded ? c->rec_id : 0;
     return VT_OK;
 }
 

@@ -44,6 +44,7 @@ struct vt_ctx {
     int rec_steps = 0, rec_every = 0;
     double rec_dt = 0.0;
     bool recorded = false;
+    unsigned long long rec_id = 0;           // number of vt_adjoint_forward calls so far
 };
 
 namespace vti {

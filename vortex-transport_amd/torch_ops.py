@@ -6,7 +6,10 @@ counterpart: the reference computes no derivatives).
                                       backward through vt_vortex_tangent
   rk4_solve(U0, solver, dt, nsteps)   U_T after nsteps RK4 steps (Timestepping.cpp:85-305)
                                       on the device; backward = the checkpointed
-                                      reverse sweep (vt_adjoint_forward/backward)
+                                      reverse sweep (vt_adjoint_forward/backward);
+                                      several calls on one solver may share a
+                                      graph: a node whose recording was replaced
+                                      records its run again before its sweep
   mass_l2(U, solver, rho_target)      J = sum_e detJ_e drho^T M_ref drho in torch
 
 Tensors are float64 of shape [n][Np][4] in the reference's element-major
@@ -33,14 +36,28 @@ class VortexState(torch.autograd.Function):
 class RK4Solve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U0, solver, dt, nsteps, every):
-        solver.set_state(U0.detach().cpu().double().numpy())
+        u0 = U0.detach().cpu().double().numpy().copy()
+        solver.set_state(u0)
         solver.adjoint_forward(dt, nsteps, every)
-        ctx.solver = solver
+        ctx.solver, ctx.u0, ctx.run = solver, u0, (dt, nsteps, every)
+        ctx.rec = solver.adjoint_recording()
         return torch.from_numpy(solver.state()).to(U0.device)
 
     @staticmethod
     def backward(ctx, g):
-        lam0 = ctx.solver.adjoint_backward(g.detach().cpu().double().numpy())
+        s = ctx.solver
+        if s.adjoint_recording() != ctx.rec:
+            # Another run was recorded on this solver since (a second rk4_solve
+            # in the same graph, or an adjoint call): record this node's run
+            # again from its saved U0, leaving the solver's state and time as
+            # they were.
+            cur, t = s.state(), s.time
+            s.set_state(ctx.u0)
+            s.adjoint_forward(*ctx.run)
+            ctx.rec = s.adjoint_recording()
+            s.set_state(cur)
+            s.time = t
+        lam0 = s.adjoint_backward(g.detach().cpu().double().numpy())
         return torch.from_numpy(lam0).to(g.device), None, None, None, None
 
 
