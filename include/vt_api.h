@@ -141,6 +141,20 @@ int vt_stage_kernel_times(vt_ctx *ctx, double dt, int nsteps, double *ms_per_sta
  * the reference's error number. */
 int vt_numerical_flux(const double *records, double *flux /* [n][4] */, int *codes, int n);
 
+/* Most records one vt_numerical_flux_noalloc call takes. */
+#define VT_NOALLOC_MAX_RECORDS 32768
+
+/* vt_numerical_flux without any allocation: no reference counterpart.  For
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  Inputs, outputs and codes are staged through one
+ * static device array of 3.8 MB (32768 records; each GPU the library runs on
+ * holds it until the process ends) instead of per-call buffers; calls no
+ * hipMalloc or hipFree.  n above VT_NOALLOC_MAX_RECORDS (or negative) returns
+ * VT_ERR_ARG before any device call.  Calls that overlap on one GPU share the
+ * array and race; calls from one thread, or serialised, do not.  Same results
+ * as vt_numerical_flux. */
+int vt_numerical_flux_noalloc(const double *records, double *flux /* [n][4] */, int *codes, int n);
+
 /* L2 state-error metric of plottools/l2stateerror.py:31-99 between the current
  * state and U_ref [n][Np][4] (the script's first dump): err[k] =
  * sqrt(I_k / 25), I_k = sum_e detJ_e sum_i w_i sum_j phi_j(x_i) (U - U_ref)_jk^2
