@@ -43,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_ELEM_STEP = 5120.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_stage_kernel_p3_512x512.txt")
 FP64_FILE = os.path.join(ROOT, "profiles", "pmc_fp64_stage_kernel_p3_512x512.txt")
+TRACE_FILE = os.path.join(ROOT, "profiles", "stage_kernel_p3_512x512_kernel_trace.md")
 FP64_PEAK_TFLOPS = 78.6
 # Algorithmic FP64 operations per element per stage at p = 3 (traces, 15 Roe
 # fluxes, lift, volume, nodal flux, residual, M^-1, update; FMA = 2).
@@ -129,6 +130,21 @@ def traffic_from_profile(n_elem):
     """HBM bytes per stage launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), scaled by the copy-kernel calibration."""
     return profile_value(TRAFFIC_FILE, "bytes_per_launch_avg", n_elem)
+
+
+def trace_avg_launch_ms(n_elem, block):
+    """Average duration of vt::stage_kernel<3, 5, block> in the committed
+    rocprofv3 --kernel-trace --stats summary, or None when the summary is
+    absent or for another mesh size."""
+    try:
+        txt = open(TRACE_FILE).read()
+    except OSError:
+        return None
+    e = re.search(r"\(([0-9,]+) triangles\)", txt)
+    m = re.search(r"stage_kernel<3, 5, %d>\(vt::StageParams\)` \| [0-9,]+ \| ([0-9,]+) \|" % block, txt)
+    if not e or not m or int(e.group(1).replace(",", "")) != n_elem:
+        return None
+    return float(m.group(1).replace(",", "")) * 1e-6
 
 
 def batched_figure(vt, steps, dt=1e-4):
@@ -278,9 +294,19 @@ def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
     meshes = [vt.generate_mesh(nx=d["cells"], ny=d["cells"]) for d in draws]
     eps, x0, y0 = ([d[k] for d in draws] for k in ("epsilon", "x0", "y0"))
     b = vt.Batch(meshes, 3, 8)
+    c0 = time.perf_counter()
     b.initialize_hydrodinamics(0, eps, x0, y0)
-    U0 = [b.state(i) for i in range(b.n)]
+    c1 = time.perf_counter()
     tg = [b.exact_vortex(i, dt * steps, 0.32, 0.1, -0.05)[:, :, 0] for i in range(b.n)]
+    c2 = time.perf_counter()
+    del tg
+    # the run itself starts from the device set-up: initial states and the
+    # objectives' target vortices computed on the device
+    b.initialize_hydrodinamics(0, eps, x0, y0, device=True)
+    b.set_target_vortex(dt * steps, [0.32] * b.n, [0.1] * b.n, [-0.05] * b.n)
+    torch.cuda.synchronize()
+    c3 = time.perf_counter()
+    U0 = [b.state(i) for i in range(b.n)]
     b.rk4(dt, 4)
     b.prepare(1, dt, steps)
     for i in range(b.n):
@@ -294,7 +320,7 @@ def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
         b.set_state(i, U0[i])
     b.time = 0.0
     t0 = time.perf_counter()
-    J, grad = b.adjoint_vortex(dt, steps, tg, eps, x0, y0, every=every)
+    J, grad = b.adjoint_vortex(dt, steps, None, eps, x0, y0, every=every)
     ta = time.perf_counter() - t0
     ne = sum(b.ne)
     slot_bytes = ne * 10 * 4 * 8
@@ -306,6 +332,8 @@ def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
             "forward_plus_adjoint_s": ta, "ratio": ta / tf, "launches_per_stage": 1,
             "parameter_gradient": "dJ_i/d(eps_i, x0_i, y0_i) contracted on the device",
             "device_state_gb": slots * slot_bytes / 1e9, "J_sum": float(np.sum(J)),
+            "setup_s": {"initial_states_host": c1 - c0, "targets_host": c2 - c1,
+                        "initial_states_and_targets_device": c3 - c2},
             "grad_finite": bool(np.isfinite(grad).all()), "grad_abs_max": float(np.abs(grad).max())}
 
 
@@ -339,6 +367,10 @@ def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
     c2 = time.perf_counter()
     s.initialize_hydrodinamics(0)
     c3 = time.perf_counter()
+    s.initialize_hydrodinamics(0, device=True)
+    torch.cuda.synchronize()
+    c4 = time.perf_counter()
+    s.initialize_hydrodinamics(0)
     s.rk4(dt, 4)
     s.prepare(1, dt, steps)
     settle_clocks(s, dt, 0.2)
@@ -355,7 +387,8 @@ def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
             "elements": ne, "steps": steps, "ms_per_step": wall / steps * 1e3,
             "value": ne * 10 * 4 * 4 * steps / wall, "unit": "DOF-stage updates/s",
             "state_bytes": ne * 10 * 4 * 8, "device_used_gb": (total - free) / 1e9, "finite": ok,
-            "setup_s": {"mesh": c1 - c0, "create": c2 - c1, "initial_state": c3 - c2}}
+            "setup_s": {"mesh": c1 - c0, "create": c2 - c1, "initial_state": c3 - c2,
+                        "initial_state_device": c4 - c3}}
 
 
 def pcie_figure(vt, steps, dt=1e-4, n=512):
@@ -542,6 +575,7 @@ def main():
     bytes_per_launch = ne * BYTES_PER_ELEM_STEP / 4
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = traffic_from_profile(ne)
+    trace_ms = trace_avg_launch_ms(ne, block)
     fp64_ops = profile_value(FP64_FILE, "fp64_ops_per_launch_avg", ne)
 
     dof_stage = ne * npn * 4 * 4
@@ -570,6 +604,8 @@ def main():
                      "kernel": "vt::stage_kernel<3,5,%d> (4 launches per step)" % block,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3,
+                     "rocprof_avg_launch_ms": trace_ms,
+                     "frac_rocprof": None if trace_ms is None else bytes_per_launch / (trace_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "stage_ms_single_launches": [float(x) for x in stage_ms]},
         "fp64": {"algorithmic": FLOPS_PER_ELEM_STAGE * ne / avg_launch_s / 1e12,
                  "executed": None if fp64_ops is None else fp64_ops / avg_launch_s / 1e12,

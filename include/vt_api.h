@@ -97,6 +97,18 @@ int vt_num_nodes(const vt_ctx *ctx);
 int vt_init_vortex(vt_ctx *ctx, int init_type, double epsilon, double x0, double y0);
 /* Exact vortex at the nodes at time t (the init_type 0 formula with t > 0). */
 int vt_exact_vortex(const vt_ctx *ctx, double t, double epsilon, double x0, double y0, double *U);
+/* vt_init_vortex computed on the device (Element.cpp:119-326 as the set-up of
+ * large meshes and ensembles; the reference computes it on the host): no host
+ * state array and no transfer.  The formula's operations are those of
+ * vt_init_vortex in the same order and rounding except exp and pow, which are
+ * the device library's, so values can differ from vt_init_vortex's (which
+ * equal the reference's bits) in the last ulps. */
+int vt_init_vortex_device(vt_ctx *ctx, int init_type, double epsilon, double x0, double y0);
+/* Sets the objective's target to the nodal density of the type 0 vortex
+ * (epsilon, x0, y0) at time t, computed on the device when the objective is
+ * evaluated: vt_mass_l2_objective and vt_adjoint then accept rho_target = NULL
+ * (no reference counterpart). */
+int vt_set_target_vortex(vt_ctx *ctx, double t, double epsilon, double x0, double y0);
 
 int vt_set_state(vt_ctx *ctx, const double *U);
 int vt_get_state(vt_ctx *ctx, double *U);
@@ -198,7 +210,8 @@ int vt_adjoint_backward(vt_ctx *ctx, const double *lamT, double *lam0);
 int vt_vortex_tangent(const vt_ctx *ctx, double t, double epsilon, double x0, double y0, double *dU);
 /* J = sum_e detJ_e (rho_e - rho_target_e)^T M_ref (rho_e - rho_target_e) of the
  * current state, M_ref the reference mass matrix of Preevolve.cpp:43-54;
- * rho_target [n][Np]; dJdU [n][Np][4] may be NULL. */
+ * rho_target [n][Np], or NULL for the target of vt_set_target_vortex (else
+ * VT_ERR_ARG); dJdU [n][Np][4] may be NULL. */
 int vt_mass_l2_objective(vt_ctx *ctx, const double *rho_target, double *J, double *dJdU);
 /* vt_adjoint_forward, then J and dJ/dU_T of vt_mass_l2_objective at U_T, then
  * vt_adjoint_backward: dJdU0 = dJ/dU_0. */
@@ -223,6 +236,14 @@ int vt_batch_num_instances(const vt_batch *b);
 int vt_batch_num_launches(const vt_batch *b);
 /* Vortex initial state per instance (vt_init_vortex with epsilon[i], x0[i], y0[i]). */
 int vt_batch_init_vortex(vt_batch *b, int init_type, const double *epsilon, const double *x0, const double *y0);
+/* The same computed on the device, one launch per instance
+ * (vt_init_vortex_device). */
+int vt_batch_init_vortex_device(vt_batch *b, int init_type, const double *epsilon, const double *x0,
+                                const double *y0);
+/* Target of every instance's objective: the type 0 vortex (epsilon[i], x0[i],
+ * y0[i]) at time t, computed on the device when the objectives are evaluated;
+ * the ensemble objective and adjoint calls then accept rho_target = NULL. */
+int vt_batch_set_target_vortex(vt_batch *b, double t, const double *epsilon, const double *x0, const double *y0);
 int vt_batch_set_state(vt_batch *b, int inst, const double *U /* [n_elem[inst]][Np][4] */);
 int vt_batch_get_state(vt_batch *b, int inst, double *U);
 double vt_batch_get_time(const vt_batch *b);
@@ -249,7 +270,8 @@ int vt_batch_exact_vortex(const vt_batch *b, int inst, double t, double epsilon,
  * and gradient dJ_i/dU0_i.  Instances with the same (p, integration order)
  * share every launch of the forward, objective and reverse sweep, as they
  * share the launches of vt_batch_step.  Concatenated arrays list the instances
- * in order: rho_target [sum n_i][Np_i], dJdU [sum n_i][Np_i][4]. */
+ * in order: rho_target [sum n_i][Np_i] (or NULL for the targets of
+ * vt_batch_set_target_vortex), dJdU [sum n_i][Np_i][4]. */
 /* J [n_inst] and, when dJdU is not NULL, dJ_i/dU_i at the current state. */
 int vt_batch_mass_l2_objective(vt_batch *b, const double *rho_target, double *J, double *dJdU);
 /* nsteps RK4 steps of dt recorded with a checkpoint every `every` steps

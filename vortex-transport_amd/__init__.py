@@ -84,6 +84,12 @@ def lib():
             "vt_num_elements": (ctypes.c_int, [vp]),
             "vt_num_nodes": (ctypes.c_int, [vp]),
             "vt_init_vortex": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+            "vt_init_vortex_device": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                     ctypes.c_double]),
+            "vt_set_target_vortex": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double]),
+            "vt_batch_init_vortex_device": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp]),
+            "vt_batch_set_target_vortex": (ctypes.c_int, [vp, ctypes.c_double, _dp, _dp, _dp]),
             "vt_exact_vortex": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
             "vt_set_state": (ctypes.c_int, [vp, _dp]),
             "vt_get_state": (ctypes.c_int, [vp, _dp]),
@@ -345,8 +351,19 @@ class Solver:
     def __del__(self):
         self.close()
 
-    def initialize_hydrodinamics(self, init_type=0, epsilon=0.3, x0=0.0, y0=0.0):
-        _check(lib().vt_init_vortex(self._h, init_type, epsilon, x0, y0), self)
+    def initialize_hydrodinamics(self, init_type=0, epsilon=0.3, x0=0.0, y0=0.0, device=False):
+        """Vortex initial state (Element::initialize_hydrodinamics).  device:
+        computed by a device kernel instead of on the host (the host form has
+        the reference's bits; the device form differs in the last ulps where
+        the device exp / pow round differently)."""
+        f = lib().vt_init_vortex_device if device else lib().vt_init_vortex
+        _check(f(self._h, init_type, epsilon, x0, y0), self)
+
+    def set_target_vortex(self, t, epsilon=0.3, x0=0.0, y0=0.0):
+        """Objective target = nodal density of the vortex at time t, computed
+        on the device; mass_l2_objective() and adjoint() then take
+        rho_target=None."""
+        _check(lib().vt_set_target_vortex(self._h, t, epsilon, x0, y0))
 
     def exact_vortex(self, t, epsilon=0.3, x0=0.0, y0=0.0):
         U = np.zeros((self.ne, self.np, 4))
@@ -448,20 +465,25 @@ class Solver:
         _check(lib().vt_l2_state_error(self._h, _d(U), _d(err)))
         return err
 
-    def mass_l2_objective(self, rho_target):
-        """(J, dJ/dU) of J = sum_e detJ_e drho^T M_ref drho at the current state."""
-        rt = np.ascontiguousarray(rho_target, dtype=np.float64).reshape(self.ne, self.np)
+    def _target(self, rho_target):
+        if rho_target is None:
+            return None  # the device target of set_target_vortex
+        return _d(np.ascontiguousarray(rho_target, dtype=np.float64).reshape(self.ne, self.np))
+
+    def mass_l2_objective(self, rho_target=None):
+        """(J, dJ/dU) of J = sum_e detJ_e drho^T M_ref drho at the current
+        state; rho_target None: the target of set_target_vortex()."""
         J = ctypes.c_double()
         g = np.zeros((self.ne, self.np, 4))
-        _check(lib().vt_mass_l2_objective(self._h, _d(rt), ctypes.byref(J), _d(g)))
+        _check(lib().vt_mass_l2_objective(self._h, self._target(rho_target), ctypes.byref(J), _d(g)))
         return J.value, g
 
-    def adjoint(self, dt, nsteps, rho_target, every=32):
-        """Forward run, objective and reverse sweep: (J, dJ/dU0)."""
-        rt = np.ascontiguousarray(rho_target, dtype=np.float64).reshape(self.ne, self.np)
+    def adjoint(self, dt, nsteps, rho_target=None, every=32):
+        """Forward run, objective and reverse sweep: (J, dJ/dU0); rho_target
+        None: the target of set_target_vortex()."""
         J = ctypes.c_double()
         g = np.zeros((self.ne, self.np, 4))
-        _check(lib().vt_adjoint(self._h, dt, nsteps, every, _d(rt), ctypes.byref(J), _d(g)), self)
+        _check(lib().vt_adjoint(self._h, dt, nsteps, every, self._target(rho_target), ctypes.byref(J), _d(g)), self)
         return J.value, g
 
     def launch(self):
@@ -522,11 +544,22 @@ class Batch:
     def num_instances(self):
         return lib().vt_batch_num_instances(self._h)
 
-    def initialize_hydrodinamics(self, init_type, epsilon, x0, y0):
-        e = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (self.n,)))
-        a = np.ascontiguousarray(np.broadcast_to(np.asarray(x0, dtype=np.float64), (self.n,)))
-        c = np.ascontiguousarray(np.broadcast_to(np.asarray(y0, dtype=np.float64), (self.n,)))
-        _check(lib().vt_batch_init_vortex(self._h, init_type, _d(e), _d(a), _d(c)), self)
+    def _per_instance(self, *vals):
+        return [np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (self.n,))) for v in vals]
+
+    def initialize_hydrodinamics(self, init_type, epsilon, x0, y0, device=False):
+        """Vortex initial state of every instance; device: computed by a
+        device kernel (Solver.initialize_hydrodinamics)."""
+        e, a, c = self._per_instance(epsilon, x0, y0)
+        f = lib().vt_batch_init_vortex_device if device else lib().vt_batch_init_vortex
+        _check(f(self._h, init_type, _d(e), _d(a), _d(c)), self)
+
+    def set_target_vortex(self, t, epsilon, x0, y0):
+        """Every instance's objective target = nodal density of its vortex
+        (epsilon[i], x0[i], y0[i]) at time t, computed on the device; the
+        objective and adjoint calls then take rho_targets=None."""
+        e, a, c = self._per_instance(epsilon, x0, y0)
+        _check(lib().vt_batch_set_target_vortex(self._h, t, _d(e), _d(a), _d(c)))
 
     def _inst(self, i):
         if not 0 <= i < self.n:
@@ -596,7 +629,12 @@ class Batch:
         _check(lib().vt_batch_exact_vortex(self._h, i, t, epsilon, x0, y0, _d(U)))
         return U
 
+    def _nodes(self):
+        return sum(self.ne[i] * self.np[i] for i in range(self.n))
+
     def _targets(self, rho_targets):
+        if rho_targets is None:
+            return None  # the device targets of set_target_vortex
         if len(rho_targets) != self.n:
             raise ValueError("one target density per instance")
         return np.ascontiguousarray(np.concatenate(
@@ -610,36 +648,36 @@ class Batch:
             o += k
         return out
 
-    def mass_l2_objective(self, rho_targets):
+    def mass_l2_objective(self, rho_targets=None):
         """(J [n], [dJ_i/dU_i]) at the current states; rho_targets: one
-        [n_i][Np_i] density per instance."""
+        [n_i][Np_i] density per instance, or None for the targets of
+        set_target_vortex()."""
         rt = self._targets(rho_targets)
         J = np.zeros(self.n)
-        g = np.empty(4 * rt.size)  # every entry is written
-        _check(lib().vt_batch_mass_l2_objective(self._h, _d(rt), _d(J), _d(g)), self)
+        g = np.empty(4 * self._nodes())  # every entry is written
+        _check(lib().vt_batch_mass_l2_objective(self._h, None if rt is None else _d(rt), _d(J), _d(g)), self)
         return J, self._split(g)
 
-    def adjoint(self, dt, nsteps, rho_targets, every=32):
+    def adjoint(self, dt, nsteps, rho_targets=None, every=32):
         """Recorded forward run of every instance, objectives at the end and
         the reverse sweep: (J [n], [dJ_i/dU0_i])."""
         rt = self._targets(rho_targets)
         J = np.zeros(self.n)
-        g = np.empty(4 * rt.size)  # every entry is written
-        _check(lib().vt_batch_adjoint(self._h, dt, nsteps, every, _d(rt), _d(J), _d(g)), self)
+        g = np.empty(4 * self._nodes())  # every entry is written
+        _check(lib().vt_batch_adjoint(self._h, dt, nsteps, every, None if rt is None else _d(rt), _d(J), _d(g)), self)
         return J, self._split(g)
 
     def adjoint_vortex(self, dt, nsteps, rho_targets, epsilon, x0, y0, every=32):
         """adjoint() and vortex_gradient() at the current time in one call,
         with dJ_i/dU0_i kept on the device: (J [n], dJ_i/d(epsilon_i, x0_i,
-        y0_i) [n][3]) for type 0 vortex initial states."""
+        y0_i) [n][3]) for type 0 vortex initial states; rho_targets None:
+        the targets of set_target_vortex()."""
         rt = self._targets(rho_targets)
-        e = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (self.n,)))
-        a = np.ascontiguousarray(np.broadcast_to(np.asarray(x0, dtype=np.float64), (self.n,)))
-        c = np.ascontiguousarray(np.broadcast_to(np.asarray(y0, dtype=np.float64), (self.n,)))
+        e, a, c = self._per_instance(epsilon, x0, y0)
         J = np.zeros(self.n)
         grad = np.zeros((self.n, 3))
-        _check(lib().vt_batch_adjoint_vortex(self._h, dt, nsteps, every, _d(rt), _d(e), _d(a), _d(c), _d(J),
-                                             _d(grad)), self)
+        _check(lib().vt_batch_adjoint_vortex(self._h, dt, nsteps, every, None if rt is None else _d(rt), _d(e), _d(a),
+                                             _d(c), _d(J), _d(grad)), self)
         return J, grad
 
     def vortex_gradient(self, t, epsilon, x0, y0, dJdU0):

@@ -39,7 +39,8 @@ int vti::ensure_adj_planes(vt_ctx* c) {
     return VT_OK;
 }
 
-int vti::objective(vt_ctx* c, int nr, const int* e0, const int* n, const double* const* rho, double* J) {
+int vti::objective(vt_ctx* c, int nr, const int* e0, const int* n, const double* const* rho, const double* vpar,
+                   double* J) {
     if (c->order < 0) return VT_ERR_QUAD;  // M_ref comes from the integration order's area rule
     int rc = ensure_adj_planes(c);
     if (rc) return rc;
@@ -60,10 +61,17 @@ int vti::objective(vt_ctx* c, int nr, const int* e0, const int* n, const double*
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->part), sizeof(double) * boff[nr]));
         c->part_cap = boff[nr];
     }
-    double* h = host_stage(c, std::max(static_cast<size_t>(c->np) * c->ld, static_cast<size_t>(boff[nr])));
+    double* h = host_stage(c, rho ? std::max(static_cast<size_t>(c->np) * c->ld, static_cast<size_t>(boff[nr]))
+                                  : static_cast<size_t>(boff[nr]));
     if (!h) return VT_ERR_HIP;
-    for (int r = 0; r < nr; ++r) vt::density_to_planes(n[r], c->np, c->ld, rho[r], h + e0[r]);
-    HIP_TRY(hipMemcpyAsync(c->tgt, h, sizeof(double) * c->np * c->ld, hipMemcpyHostToDevice, c->stream));
+    if (rho) {
+        for (int r = 0; r < nr; ++r) vt::density_to_planes(n[r], c->np, c->ld, rho[r], h + e0[r]);
+        HIP_TRY(hipMemcpyAsync(c->tgt, h, sizeof(double) * c->np * c->ld, hipMemcpyHostToDevice, c->stream));
+    } else {
+        if (!vpar) return VT_ERR_ARG;
+        rc = vortex_device(c, nr, e0, n, vpar, 0, c->tgt, true);
+        if (rc) return rc;
+    }
     // One launch per element range: the plane base pointers move to the
     // range's first element, the plane stride stays ld.
     for (int r = 0; r < nr; ++r) {
@@ -276,20 +284,21 @@ int vt_vortex_tangent(const vt_ctx* c, double t, double epsilon, double x0, doub
 }
 
 int vt_mass_l2_objective(vt_ctx* c, const double* rho_target, double* J, double* dJdU) {
-    if (!c || !rho_target || !J) return VT_ERR_ARG;
+    if (!c || !J) return VT_ERR_ARG;
     const int e0 = 0;
-    int rc = vti::objective(c, 1, &e0, &c->ne, &rho_target, J);
+    if (!rho_target && !c->tgt_vortex) return VT_ERR_ARG;
+    int rc = vti::objective(c, 1, &e0, &c->ne, rho_target ? &rho_target : nullptr, c->tgt_par, J);
     if (rc || !dJdU) return rc;
     return download_planes(c, c->adj, dJdU);
 }
 
 int vt_adjoint(vt_ctx* c, double dt, int nsteps, int every, const double* rho_target, double* J, double* dJdU0) {
-    if (!c || !rho_target || !J || !dJdU0) return VT_ERR_ARG;
+    if (!c || !J || !dJdU0 || (!rho_target && !c->tgt_vortex)) return VT_ERR_ARG;
     int rc = vt_adjoint_forward(c, dt, nsteps, every);
     if (rc) return rc;
     // The seed dJ/dU_T goes from the objective kernel straight into lam.
     const int e0 = 0;
-    rc = vti::objective(c, 1, &e0, &c->ne, &rho_target, J);
+    rc = vti::objective(c, 1, &e0, &c->ne, rho_target ? &rho_target : nullptr, c->tgt_par, J);
     if (rc) return rc;
     rc = vti::backward_sweep(c);
     if (rc) return rc;

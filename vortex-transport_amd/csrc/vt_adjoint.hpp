@@ -28,10 +28,6 @@
 
 #include "vt_stage.hpp"
 
-#ifndef VT_ADJ_MIN_WAVES
-#define VT_ADJ_MIN_WAVES 1
-#endif
-
 namespace vt {
 
 // (dF/dUL)^T bF for the Roe flux of Numericalflux.cpp:31-156, differentiated on
@@ -196,7 +192,7 @@ __global__ void __launch_bounds__(256) adjoint_seed_kernel(AdjParams prm) {
 // One adjoint stage: g = f'(U)^T w for all elements, w given through y = Y,
 // then the reverse-RK4 bookkeeping of the mode (kAdjRaw: Yout = g).
 template <int P, int NQ>
-__global__ void __launch_bounds__(kAdjBlock, VT_ADJ_MIN_WAVES) adjoint_stage_kernel(AdjParams prm) {
+__global__ void __launch_bounds__(kAdjBlock, 1) adjoint_stage_kernel(AdjParams prm) {
     constexpr int NP = (P + 1) * (P + 2) / 2;
     constexpr int NE = P + 1;
     constexpr int NV = 4 * NP;
@@ -281,7 +277,7 @@ __global__ void __launch_bounds__(kAdjBlock, VT_ADJ_MIN_WAVES) adjoint_stage_ker
                 un[m][k] = plane_at(U, v, ld8, onb[s]);
                 uo[m][k] = plane_at(U, v, ld8, oe);
             }
-#pragma unroll VT_Q_UNROLL
+#pragma unroll 1
         for (int q = 0; q < NQ; ++q) {
             // d(q) = len (c_e(q) - c_nb(q~)), the cotangent of this element's
             // flux minus that of the neighbour's flux at the same point.

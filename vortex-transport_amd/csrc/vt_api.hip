@@ -338,6 +338,51 @@ int create_ctx(int n, const double* vertices, const int* neighbors, const int* t
     return VT_OK;
 }
 
+int vortex_device(vt_ctx* c, int nr, const int* e0, const int* n, const double* par, int type, double* out,
+                  bool density_only) {
+    if (type != 0 && type != 1) return VT_ERR_INIT_TYPE;
+    vt::VortexParams v{};
+    if (!vt::reference_nodes(c->p, v.nodes)) return VT_ERR_UNSUPPORTED_P;
+    v.ld = c->ld;
+    v.type = type;
+    v.density_only = density_only ? 1 : 0;
+    if (type == 1) {
+        if (c->order < 0) return VT_ERR_QUAD;  // no area rule known
+        std::vector<double> q;
+        const int nqa = vt::area_rule(c->order, &q);
+        if (nqa < 1) return VT_ERR_QUAD;
+        v.nqa = nqa;
+        if (!c->vrule) {
+            // rule [nqa][3], phi [Np][nqa], Minv_ref [Np][Np]
+            const int np = c->np;
+            std::vector<double> t(static_cast<size_t>(3 + np) * nqa + static_cast<size_t>(np) * np);
+            std::copy(q.begin(), q.end(), t.begin());
+            for (int k = 0; k < nqa; ++k) {
+                double b[vt::kMaxNp];
+                vt::basis(c->p, q[3 * k], q[3 * k + 1], b);
+                for (int j = 0; j < np; ++j) t[3 * static_cast<size_t>(nqa) + static_cast<size_t>(j) * nqa + k] = b[j];
+            }
+            if (!vt::reference_operators(c->p, c->order, nullptr, t.data() + static_cast<size_t>(3 + np) * nqa, nullptr))
+                return VT_ERR_QUAD;
+            HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->vrule), sizeof(double) * t.size()));
+            HIP_TRY(hipMemcpyAsync(c->vrule, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+        }
+        v.rule = c->vrule;
+    }
+    for (int r = 0; r < nr; ++r) {
+        v.out = out + e0[r];
+        v.vtx = c->vtx + e0[r];
+        v.ne = n[r];
+        v.t = par[4 * r];
+        vt::vortex_constants(par[4 * r + 1], v.k);
+        v.x0 = par[4 * r + 2];
+        v.y0 = par[4 * r + 3];
+        HIP_TRY(vt::launch_vortex_state(v, c->p, c->stream));
+    }
+    return VT_OK;
+}
+
 }  // namespace vti
 
 extern "C" {
@@ -372,7 +417,7 @@ void vt_destroy(vt_ctx* c) {
     for (void* b : {static_cast<void*>(c->Un), static_cast<void*>(c->Ua), static_cast<void*>(c->Ub),
                     static_cast<void*>(c->acc), static_cast<void*>(c->vtx), static_cast<void*>(c->nbr),
                     static_cast<void*>(c->tab), static_cast<void*>(c->err), static_cast<void*>(c->l2ref),
-                    static_cast<void*>(c->l2tab), static_cast<void*>(c->l2part)})
+                    static_cast<void*>(c->l2tab), static_cast<void*>(c->l2part), static_cast<void*>(c->vrule)})
         if (b) hipFree(b);
     if (c->err_init) hipHostFree(c->err_init);
     if (c->err_host) hipHostFree(c->err_host);
@@ -391,6 +436,26 @@ int vt_init_vortex(vt_ctx* c, int init_type, double epsilon, double x0, double y
     if (!vt::initial_state(c->ne, c->vertices.data(), c->p, c->order, init_type, epsilon, x0, y0, c->time, U.data()))
         return VT_ERR_INIT_TYPE;
     return vt_set_state(c, U.data());
+}
+
+int vt_init_vortex_device(vt_ctx* c, int init_type, double epsilon, double x0, double y0) {
+    if (!c) return VT_ERR_ARG;
+    const int e0 = 0;
+    const double par[4] = {c->time, epsilon, x0, y0};
+    const int rc = vti::vortex_device(c, 1, &e0, &c->ne, par, init_type, c->Un, false);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VT_OK;
+}
+
+int vt_set_target_vortex(vt_ctx* c, double t, double epsilon, double x0, double y0) {
+    if (!c) return VT_ERR_ARG;
+    c->tgt_par[0] = t;
+    c->tgt_par[1] = epsilon;
+    c->tgt_par[2] = x0;
+    c->tgt_par[3] = y0;
+    c->tgt_vortex = true;
+    return VT_OK;
 }
 
 int vt_exact_vortex(const vt_ctx* c, double t, double epsilon, double x0, double y0, double* U) {

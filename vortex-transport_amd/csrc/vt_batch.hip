@@ -37,6 +37,7 @@ struct vt_batch {
     std::map<std::tuple<int, unsigned long long, int>, hipGraphExec_t> graphs;
     double time = 0.0;
     int last_err[7] = {-1, -1, -1, -1, 0, 0, 0};
+    std::vector<double> tgt_par;               // [n][4] target vortex {t, eps, x0, y0}, or empty
 };
 
 namespace {
@@ -201,19 +202,25 @@ std::vector<size_t> node_offsets(const vt_batch* b) {
 
 // Objective of every instance of group gi at its current state: J of each
 // member, dJ/dU of the group in its lam planes.
+// rho_target null: the members' target vortices set by vt_batch_set_target_vortex,
+// computed on the device.
 int group_objective(vt_batch* b, int gi, const double* rho_target, const std::vector<size_t>& off, double* J) {
     vt_ctx* c = b->g[gi];
     const std::vector<int>& mem = b->members[gi];
     std::vector<int> e0(mem.size()), n(mem.size());
     std::vector<const double*> rho(mem.size());
+    std::vector<double> par(4 * mem.size());
+    if (!rho_target && b->tgt_par.empty()) return VT_ERR_ARG;
     for (size_t k = 0; k < mem.size(); ++k) {
         const int i = mem[k];
         e0[k] = b->eoff[i];
         n[k] = b->ne[i];
-        rho[k] = rho_target + off[i];
+        if (rho_target) rho[k] = rho_target + off[i];
+        else std::copy(b->tgt_par.begin() + 4 * i, b->tgt_par.begin() + 4 * i + 4, par.begin() + 4 * k);
     }
     std::vector<double> Jg(mem.size());
-    const int rc = vti::objective(c, static_cast<int>(mem.size()), e0.data(), n.data(), rho.data(), Jg.data());
+    const int rc = vti::objective(c, static_cast<int>(mem.size()), e0.data(), n.data(), rho_target ? rho.data() : nullptr,
+                                  par.data(), Jg.data());
     if (rc) return rc;
     for (size_t k = 0; k < mem.size(); ++k) J[mem[k]] = Jg[k];
     return VT_OK;
@@ -418,6 +425,42 @@ int vt_batch_init_vortex(vt_batch* b, int init_type, const double* eps, const do
     return VT_OK;
 }
 
+int vt_batch_init_vortex_device(vt_batch* b, int init_type, const double* eps, const double* x0, const double* y0) {
+    if (!b || !eps || !x0 || !y0) return VT_ERR_ARG;
+    for (int gi = 0; gi < static_cast<int>(b->g.size()); ++gi) {
+        vt_ctx* c = b->g[gi];
+        const std::vector<int>& mem = b->members[gi];
+        std::vector<int> e0(mem.size()), n(mem.size());
+        std::vector<double> par(4 * mem.size());
+        for (size_t k = 0; k < mem.size(); ++k) {
+            const int i = mem[k];
+            e0[k] = b->eoff[i];
+            n[k] = b->ne[i];
+            par[4 * k] = b->time;
+            par[4 * k + 1] = eps[i];
+            par[4 * k + 2] = x0[i];
+            par[4 * k + 3] = y0[i];
+        }
+        int rc = vti::vortex_device(c, static_cast<int>(mem.size()), e0.data(), n.data(), par.data(), init_type, c->Un,
+                                    false);
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return VT_OK;
+}
+
+int vt_batch_set_target_vortex(vt_batch* b, double t, const double* eps, const double* x0, const double* y0) {
+    if (!b || !eps || !x0 || !y0) return VT_ERR_ARG;
+    b->tgt_par.resize(4 * static_cast<size_t>(b->n));
+    for (int i = 0; i < b->n; ++i) {
+        b->tgt_par[4 * i] = t;
+        b->tgt_par[4 * i + 1] = eps[i];
+        b->tgt_par[4 * i + 2] = x0[i];
+        b->tgt_par[4 * i + 3] = y0[i];
+    }
+    return VT_OK;
+}
+
 int vt_batch_set_state(vt_batch* b, int i, const double* U) {
     if (check_inst(b, i) || !U) return VT_ERR_ARG;
     vt_ctx* c = b->g[b->grp[i]];
@@ -574,7 +617,7 @@ int vt_batch_exact_vortex(const vt_batch* b, int i, double t, double epsilon, do
 }
 
 int vt_batch_mass_l2_objective(vt_batch* b, const double* rho_target, double* J, double* dJdU) {
-    if (!b || !rho_target || !J) return VT_ERR_ARG;
+    if (!b || !J) return VT_ERR_ARG;
     const std::vector<size_t> off = node_offsets(b);
     for (int gi = 0; gi < static_cast<int>(b->g.size()); ++gi) {
         int rc = group_objective(b, gi, rho_target, off, J);
@@ -586,7 +629,7 @@ int vt_batch_mass_l2_objective(vt_batch* b, const double* rho_target, double* J,
 
 int vt_batch_adjoint(vt_batch* b, double dt, int nsteps, int every, const double* rho_target, double* J,
                      double* dJdU0) {
-    if (!b || !rho_target || !J || !dJdU0 || nsteps < 1 || every < 1) return VT_ERR_ARG;
+    if (!b || !J || !dJdU0 || nsteps < 1 || every < 1 || (!rho_target && b->tgt_par.empty())) return VT_ERR_ARG;
     return batch_adjoint(b, dt, nsteps, every, rho_target, J, [&](int gi, const std::vector<size_t>& off) {
         return scatter_group_gradient(b, gi, off, dJdU0);
     });
@@ -594,7 +637,8 @@ int vt_batch_adjoint(vt_batch* b, double dt, int nsteps, int every, const double
 
 int vt_batch_adjoint_vortex(vt_batch* b, double dt, int nsteps, int every, const double* rho_target,
                             const double* eps, const double* x0, const double* y0, double* J, double* grad) {
-    if (!b || !rho_target || !eps || !x0 || !y0 || !J || !grad || nsteps < 1 || every < 1) return VT_ERR_ARG;
+    if (!b || !eps || !x0 || !y0 || !J || !grad || nsteps < 1 || every < 1 || (!rho_target && b->tgt_par.empty()))
+        return VT_ERR_ARG;
     const double t0 = b->time;
     return batch_adjoint(b, dt, nsteps, every, rho_target, J, [&](int gi, const std::vector<size_t>&) {
         return group_vortex_gradient(b, gi, t0, eps, x0, y0, grad);

@@ -45,6 +45,12 @@ struct vt_ctx {
     double rec_dt = 0.0;
     bool recorded = false;
     unsigned long long rec_id = 0;           // number of vt_adjoint_forward calls so far
+    // Device set-up of the vortex state: area rule, basis values and Minv_ref
+    // of the least-squares type (uploaded on first use), and the objective's
+    // target vortex {t, eps, x0, y0} set by vt_set_target_vortex.
+    double* vrule = nullptr;
+    bool tgt_vortex = false;
+    double tgt_par[4] = {0.0, 0.0, 0.0, 0.0};
 };
 
 namespace vti {
@@ -98,10 +104,18 @@ void free_adjoint(vt_ctx* c);
 int ensure_adj_planes(vt_ctx* c);
 
 // Objective of the adjoint over nr element ranges [e0[r], e0[r] + n[r]) of the
-// current state, with target densities rho[r] [n[r]][Np]: J[r] per range
-// (block partial sums added in block order) and dJ/dU of every range written
-// into the lam planes.
-int objective(vt_ctx* c, int nr, const int* e0, const int* n, const double* const* rho, double* J);
+// current state, with target densities rho[r] [n[r]][Np] from the host, or,
+// with rho null, the nodal density of the vortex vpar[r] = {t, eps, x0, y0}
+// computed on the device: J[r] per range (block partial sums added in block
+// order) and dJ/dU of every range written into the lam planes.
+int objective(vt_ctx* c, int nr, const int* e0, const int* n, const double* const* rho, const double* vpar, double* J);
+
+// The vortex state on the device for element ranges [e0[r], e0[r] + n[r])
+// with parameters par[r] = {t, eps, x0, y0}: the type 0 or type 1 state into
+// the state planes `out`, or, with density_only, the type 0 nodal density
+// into the Np planes `out`.
+int vortex_device(vt_ctx* c, int nr, const int* e0, const int* n, const double* par, int type, double* out,
+                  bool density_only);
 
 // The reverse sweep over the recorded forward run, from lam = dJ/dU_T in the
 // lam planes to lam = dJ/dU_0 in the same planes.

@@ -128,6 +128,23 @@ struct VGradParams {
     int ne, ld;
 };
 
+// Isentropic-vortex state computed on the device (Element.cpp:119-326): per
+// element the type 0 nodal values, or the type 1 least-squares projection
+// over the area rule, written into state planes; with density_only, only the
+// nodal density into Np planes (an objective's target).  k holds the
+// point-independent factors of the host formula (vt::vortex_constants).
+constexpr int kVortexBlock = 128;
+
+struct VortexParams {
+    double* out;          // planes [4*Np][ld] (or [Np][ld] with density_only), element 0 of the range
+    const double* vtx;    // vertex planes, element 0 of the range
+    const double* rule;   // type 1: rule [nqa][3] (xi, eta, w), phi [Np][nqa], Minv_ref [Np][Np]
+    double nodes[2 * 10];  // reference nodes (xi, eta)
+    double k[8];
+    double t, x0, y0;
+    int ne, ld, type, nqa, density_only;
+};
+
 hipError_t launch_stage(const StageParams& prm, int p, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st);
 hipError_t launch_batched(const BatchParams& bp, int nq, const LaunchConfig& cfg, hipStream_t st);
@@ -135,6 +152,7 @@ hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStre
 hipError_t launch_objective(const ObjParams& prm, int p, hipStream_t st);
 hipError_t launch_l2(const L2Params& prm, int p, hipStream_t st);
 hipError_t launch_vortex_grad(const VGradParams& prm, int p, hipStream_t st);
+hipError_t launch_vortex_state(const VortexParams& prm, int p, hipStream_t st);
 
 inline int tiles_per_xcd(int ne, int block) { return ((ne + block - 1) / block + 7) / 8; }
 inline int blocks_per_xcd(int ne) { return tiles_per_xcd(ne, kBlock); }

@@ -170,6 +170,90 @@ __global__ void __launch_bounds__(kVGradBlock) vortex_grad_kernel(VGradParams pr
     if (threadIdx.x < 3) prm.part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x * kVGradBlock];
 }
 
+// The vortex of Element.cpp:134-160 at (x, y, t): rho, rho u, rho v, rho E,
+// operation by operation as vt::vortex_point computes it on the host (the same
+// operand order, contraction off, so each step rounds the same); exp and pow
+// come from the device library (an ulp or so from the host's) and pow(a, 2.0)
+// is a * a.
+__device__ __forceinline__ void vortex_at(const VortexParams& prm, double x, double y, double (&U)[4]) {
+#pragma clang fp contract(off)
+    const double Uinf = prm.k[0], Vinf = prm.k[1];
+    const double dx = x - prm.x0 - Uinf * prm.t, dy = y - prm.y0 - Vinf * prm.t;
+    const double f0 = 1.0 - (dx * dx + dy * dy) / 1.0;
+    const double f1 = 1.0 - prm.k[2] * exp(f0) / prm.k[3];
+    const double f2 = prm.k[4] * exp(f0 / 2.0) / prm.k[5];
+    const double rho = 1.0 * pow(f1, prm.k[6]);
+    const double u = Uinf - f2 * dy;
+    const double v = Vinf + f2 * dx;
+    const double p = 20.0 / 7.0 * pow(f1, prm.k[7]);
+    const double E = p / (rho * (1.4 - 1.0)) + (u * u + v * v) / 2.0;
+    U[0] = rho;
+    U[1] = rho * u;
+    U[2] = rho * v;
+    U[3] = rho * E;
+}
+
+template <int P>
+__global__ void __launch_bounds__(kVortexBlock) vortex_state_kernel(VortexParams prm) {
+#pragma clang fp contract(off)
+    constexpr int NP = (P + 1) * (P + 2) / 2;
+    const int e = blockIdx.x * kVortexBlock + static_cast<int>(threadIdx.x);
+    if (e >= prm.ne) return;
+    const size_t ld8 = static_cast<size_t>(prm.ld) * 8;
+    const unsigned oe = static_cast<unsigned>(e) * 8u;
+    const double x0 = plane_at(prm.vtx, 0, ld8, oe), y0 = plane_at(prm.vtx, 1, ld8, oe);
+    const double x1 = plane_at(prm.vtx, 2, ld8, oe), y1 = plane_at(prm.vtx, 3, ld8, oe);
+    const double x2 = plane_at(prm.vtx, 4, ld8, oe), y2 = plane_at(prm.vtx, 5, ld8, oe);
+    if (prm.type == 0) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const double xi = prm.nodes[2 * i], eta = prm.nodes[2 * i + 1];
+            const double f = 1.0 - xi - eta;
+            double U[4];
+            vortex_at(prm, x0 * f + x1 * xi + x2 * eta, y0 * f + y1 * xi + y2 * eta, U);
+            if (prm.density_only) {
+                plane_at(prm.out, i, ld8, oe) = U[0];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) plane_at(prm.out, k * NP + i, ld8, oe) = U[k];
+            }
+        }
+        return;
+    }
+    // Least squares (Element.cpp:180-304): b_i = sum_q detJ phi_i(q) U(q) w_q,
+    // U_i = sum_j (Minv_ij / detJ) b_j, each sum in ascending order from 0.
+    const ctab_t R = const_table(prm.rule);
+    const ctab_t phi = R + 3 * prm.nqa;
+    const ctab_t Minv = phi + NP * prm.nqa;
+    const double detJ = (x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0);
+    double b[NP][4];
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) b[i][c] = 0.0;
+    for (int q = 0; q < prm.nqa; ++q) {
+        const double xi = R[3 * q], eta = R[3 * q + 1], w = R[3 * q + 2];
+        const double f = 1.0 - xi - eta;
+        double U[4];
+        vortex_at(prm, x0 * f + x1 * xi + x2 * eta, y0 * f + y1 * xi + y2 * eta, U);
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const double dp = detJ * phi[i * prm.nqa + q];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b[i][c] += dp * U[c] * w;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) s += Minv[i * NP + j] / detJ * b[j][c];
+            plane_at(prm.out, c * NP + i, ld8, oe) = s;
+        }
+}
+
 #define VT_NQ_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
 
 // The launchers (and their kernels) are instantiated in vt_kernels_nq.hip,
@@ -230,6 +314,19 @@ hipError_t launch_vortex_grad(const VGradParams& prm, int p, hipStream_t st) {
     case 1: hipLaunchKernelGGL(vortex_grad_kernel<1>, grid, block, 0, st, prm); break;
     case 2: hipLaunchKernelGGL(vortex_grad_kernel<2>, grid, block, 0, st, prm); break;
     case 3: hipLaunchKernelGGL(vortex_grad_kernel<3>, grid, block, 0, st, prm); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_vortex_state(const VortexParams& prm, int p, hipStream_t st) {
+    if (prm.ne <= 0) return hipSuccess;
+    const dim3 grid((prm.ne + kVortexBlock - 1) / kVortexBlock), block(kVortexBlock);
+    switch (p) {
+    case 0: hipLaunchKernelGGL(vortex_state_kernel<0>, grid, block, 0, st, prm); break;
+    case 1: hipLaunchKernelGGL(vortex_state_kernel<1>, grid, block, 0, st, prm); break;
+    case 2: hipLaunchKernelGGL(vortex_state_kernel<2>, grid, block, 0, st, prm); break;
+    case 3: hipLaunchKernelGGL(vortex_state_kernel<3>, grid, block, 0, st, prm); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -272,6 +272,19 @@ void vortex_point(double x, double y, double t, double epsilon, double x0, doubl
     out[0] = rho; out[1] = u; out[2] = v; out[3] = p; out[4] = E; out[5] = E + p / rho;
 }
 
+void vortex_constants(double epsilon, double* k) {
+    const double rc = 1.0, gamma = 1.4, M_infty = 0.5;
+    const double U_infty = 1.0 / std::pow(2.0, 0.5), V_infty = 1.0 / std::pow(2.0, 0.5);
+    k[0] = U_infty;
+    k[1] = V_infty;
+    k[2] = std::pow(epsilon, 2.0) * (gamma - 1) * std::pow(M_infty, 2.0);
+    k[3] = 8.0 * std::pow(M_PI, 2.0);
+    k[4] = epsilon * (std::pow(U_infty, 2.0) + std::pow(V_infty, 2.0));
+    k[5] = 2.0 * M_PI * rc;
+    k[6] = 1.0 / (gamma - 1.0);
+    k[7] = gamma / (gamma - 1.0);
+}
+
 // Lagrangebasis.cpp:150-165 applied to each reference node (Element.cpp:47-50).
 void element_nodes(int p, const double* v, double* xy) {
     double nodes[2 * kMaxNp];

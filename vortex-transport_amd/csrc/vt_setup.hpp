@@ -51,6 +51,11 @@ void side_neighbours(int ne, const int* bnd, const int* type, int* nbr_side /*[n
 
 // Isentropic vortex (Element.cpp:134-160) at one point: rho, u, v, p, E, H.
 void vortex_point(double x, double y, double t, double epsilon, double x0, double y0, double* out);
+// The point-independent factors of vortex_point, computed by the same host
+// expressions so that a device evaluation starts from the same bits:
+// k = {U_infty, V_infty, pow(eps,2)(gamma-1)pow(M,2), 8 pow(pi,2),
+//      eps (pow(U,2) + pow(V,2)), 2 pi rc, 1/(gamma-1), gamma/(gamma-1)}.
+void vortex_constants(double epsilon, double* k /*[8]*/);
 // Nodal physical coordinates of an element (Element.cpp:47-50).
 void element_nodes(int p, const double* v /*[3][2]*/, double* xy /*[Np][2]*/);
 // Initial state, type 0 (nodal) or 1 (least-squares) (Element.cpp:119-333),

@@ -26,20 +26,10 @@
 #include "vt_device.hpp"
 #include "vt_kernels.hpp"
 
-#ifndef VT_Q_UNROLL
-#define VT_Q_UNROLL 1
-#endif
-#ifndef VT_J_UNROLL
-#define VT_J_UNROLL 1
-#endif
-#ifndef VT_PREFETCH_NEXT
-#define VT_PREFETCH_NEXT 0
-#endif
-#ifndef VT_MIN_WAVES
-#define VT_MIN_WAVES 2
-#endif
-
 namespace vt {
+
+// Waves per SIMD the stage kernels are compiled for (p = 2 asks for 3).
+constexpr int kStageMinWaves = 2;
 
 template <int P>
 __device__ __forceinline__ constexpr int edge_node(int s, int m) {
@@ -298,7 +288,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     for (int i = 0; i < NP; ++i)
 #pragma unroll
         for (int k = 0; k < 4; ++k) r[i][k] = 0.0;
-#pragma unroll VT_J_UNROLL
+#pragma unroll 1
     for (int j = 0; j < NP; ++j) {
         const double rho = my[(0 * NP + j) * BS];
         const double m1 = my[(1 * NP + j) * BS];
@@ -330,13 +320,6 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     const double detJ = (x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0);
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
-        double nxt[NE][4];
-        if (VT_PREFETCH_NEXT && s < 2) {
-#pragma unroll
-            for (int m = 0; m < NE; ++m)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) nxt[m][k] = uplane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
-        }
         const int zv = opaque_zero();
         const double dx = uplane_at(prm.vtx, 2 * ((s + 1) % 3) + zv, ld8, oe) - uplane_at(prm.vtx, 2 * s + zv, ld8, oe);
         const double dy = uplane_at(prm.vtx, 2 * ((s + 1) % 3) + 1 + zv, ld8, oe) - uplane_at(prm.vtx, 2 * s + 1 + zv, ld8, oe);
@@ -368,7 +351,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
             ctp[m] = tp[m];
             ctm[m] = tm[m];
         }
-#pragma unroll VT_Q_UNROLL
+#pragma unroll 1
         for (int q = 0; q < nqe; ++q) {
             double UL[4], UR[4], F[4];
 #pragma unroll
@@ -418,7 +401,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                 for (int m = 0; m < NE; ++m)
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
-                        un[m][k] = VT_PREFETCH_NEXT ? nxt[m][k] : uplane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
+                        un[m][k] = uplane_at(Uin, k * NP + edge_node<P>(s + 1, m), ld8, onb[s + 1]);
             }
         }
     }
@@ -532,7 +515,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
 // 2.5% faster than 2 waves); p = 0, 1 reach 3 or more waves as compiled, p = 3
 // runs 2 waves (232 VGPRs).
 template <int P, int NQ, int BS>
-__global__ void __launch_bounds__(BS, (P == 2 ? 3 : VT_MIN_WAVES)) stage_kernel(StageParams prm) {
+__global__ void __launch_bounds__(BS, (P == 2 ? 3 : kStageMinWaves)) stage_kernel(StageParams prm) {
     __shared__ double su[4 * ((P + 1) * (P + 2) / 2) * BS];
     const int b = blockIdx.x;
     stage_tile<P, NQ, BS>(prm, (b & 7) * prm.blocks_per_xcd + (b >> 3), su);
@@ -541,7 +524,7 @@ __global__ void __launch_bounds__(BS, (P == 2 ? 3 : VT_MIN_WAVES)) stage_kernel(
 // Persistent form: gridDim.x (a multiple of 8) resident workgroups, each
 // looping over the tiles of its XCD's band.
 template <int P, int NQ, int BS>
-__global__ void __launch_bounds__(BS, VT_MIN_WAVES) stage_kernel_persistent(StageParams prm) {
+__global__ void __launch_bounds__(BS, kStageMinWaves) stage_kernel_persistent(StageParams prm) {
     __shared__ double su[4 * ((P + 1) * (P + 2) / 2) * BS];
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
     for (int t = slot; t < prm.blocks_per_xcd; t += slots) {
@@ -556,7 +539,7 @@ __global__ void __launch_bounds__(BS, VT_MIN_WAVES) stage_kernel_persistent(Stag
 // slots = kbegin[ngroups] every workgroup does one tile.  The tile's group
 // selects the P code path (uniform over the workgroup).
 template <int NQ, int BS>
-__global__ void __launch_bounds__(BS, VT_MIN_WAVES) batched_stage_kernel(BatchParams bp) {
+__global__ void __launch_bounds__(BS, kStageMinWaves) batched_stage_kernel(BatchParams bp) {
     __shared__ double su[4 * 10 * BS];
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, slots = gridDim.x >> 3;
     for (int k = slot; k < bp.kbegin[bp.ngroups]; k += slots) {
