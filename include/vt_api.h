@@ -182,6 +182,16 @@ int vt_l2_state_error(vt_ctx *ctx, const double *U_ref, double err[4]);
  * Evolve.cpp:7-9, has no launch shape). */
 int vt_set_launch(vt_ctx *ctx, int block, int persistent, int slots_per_xcd);
 int vt_get_launch(const vt_ctx *ctx, int *cfg /* [3] block, persistent, slots_per_xcd */);
+/* Small-mesh path of vt_step (the reference's own 8 x 8-cell case): a mesh
+ * of at most 128 elements whose sides each face back across the same side
+ * (the reference's meshes do) is advanced by up to 64 steps per launch of one
+ * workgroup with its stage state in LDS, each shared side's Roe flux evaluated
+ * once per Gauss point, and the RK update in the reference's operation order
+ * (Timestepping.cpp:85-305).  On by default where the mesh qualifies;
+ * enable = 0 returns vt_step to the per-stage kernels, enable = 1 on a mesh
+ * that does not qualify returns VT_ERR_MESH.  No reference counterpart. */
+int vt_set_small_path(vt_ctx *ctx, int enable);
+int vt_get_small_path(const vt_ctx *ctx);
 
 /* ---- discrete adjoint ---------------------------------------------------- */
 /* Derivatives of the discrete map of vt_step (method 1): no reference

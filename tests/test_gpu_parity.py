@@ -107,10 +107,19 @@ def test_rhs_matches_reference(vt, gpu, path):
     assert rel(d["U"] + dt * dU, d["U"] + dt * d["dUdt"]) < 1e-14
 
 
+def _use(s, kernels):
+    """Selects the small-mesh path or the per-stage kernels for vt_step."""
+    if kernels == "small" and not s.small_path:
+        pytest.skip("mesh too large for the small-mesh path")
+    s.small_path = kernels == "small"
+
+
+@pytest.mark.parametrize("kernels", ["small", "stage"])
 @pytest.mark.parametrize("path", _fixtures(), ids=lambda p: os.path.basename(p)[:-4])
-def test_run_matches_reference(vt, gpu, path):
+def test_run_matches_reference(vt, gpu, path, kernels):
     d = np.load(path)
     s = _solver(vt, d)
+    _use(s, kernels)
     s.initialize_hydrodinamics(int(d["init_type"]))
     s.step(int(d["stepping_method"]), float(d["dt"]), int(d["steps"]))
     U = s.state()
@@ -272,9 +281,10 @@ def test_fullsize_conservation(vt, big):
 # Where the reference's run fails (low integration orders make the p >= 2 mass
 # matrix singular or the state non-physical), the GPU must fail at the same
 # place.
+@pytest.mark.parametrize("kernels", ["small", "stage"])
 @pytest.mark.parametrize("p", [0, 1, 2, 3])
 @pytest.mark.parametrize("order", list(range(20)))
-def test_every_quadrature_size_matches_oracle(vt, oracle, gpu, p, order):
+def test_every_quadrature_size_matches_oracle(vt, oracle, gpu, p, order, kernels):
     mesh = vt.generate_mesh(nx=5, ny=4, perturb_grid=1.0)
     if p >= 1 and order <= 1:
         # one-point area rule: M_ref is exactly singular; both refuse it (the
@@ -285,6 +295,7 @@ def test_every_quadrature_size_matches_oracle(vt, oracle, gpu, p, order):
             oracle.Solver(mesh.element_coordinates, mesh.elements_at_boundary, mesh.element_type, p, order)
         return
     s = vt.Solver(mesh, p, order)
+    _use(s, kernels)
     o = oracle.Solver(mesh.element_coordinates, mesh.elements_at_boundary, mesh.element_type, p, order)
     s.initialize_hydrodinamics(0, 0.3, 0.4, -0.3)
     o.set_state(s.state())

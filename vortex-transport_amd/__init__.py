@@ -86,6 +86,8 @@ def lib():
             "vt_init_vortex": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "vt_init_vortex_device": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                      ctypes.c_double]),
+            "vt_set_small_path": (ctypes.c_int, [vp, ctypes.c_int]),
+            "vt_get_small_path": (ctypes.c_int, [vp]),
             "vt_set_target_vortex": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                     ctypes.c_double]),
             "vt_batch_init_vortex_device": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp]),
@@ -490,6 +492,16 @@ class Solver:
         c = np.zeros(3, dtype=np.int32)
         _check(lib().vt_get_launch(self._h, _i(c)))
         return {"block": int(c[0]), "persistent": bool(c[1]), "slots_per_xcd": int(c[2])}
+
+    @property
+    def small_path(self):
+        """True when rk4() / forward_euler() run the small-mesh path (one
+        workgroup, state in LDS, up to 64 steps per launch)."""
+        return bool(lib().vt_get_small_path(self._h))
+
+    @small_path.setter
+    def small_path(self, on):
+        _check(lib().vt_set_small_path(self._h, 1 if on else 0))
 
 
 # ---------------------------------------------------------------------------

@@ -243,6 +243,15 @@ int vt_adjoint_forward(vt_ctx* c, double dt, int nsteps, int every) {
             c->ckpt.push_back(p);
         }
     }
+    // The recording runs the per-stage kernels, whose stage states the reverse
+    // sweep recomputes from the checkpoints (not the small-mesh path, which
+    // rounds the RK update differently).
+    struct StageKernels {  // restores the context's choice on every return
+        vt_ctx* c;
+        bool was;
+        explicit StageKernels(vt_ctx* x) : c(x), was(x->small_on) { c->small_on = false; }
+        ~StageKernels() { c->small_on = was; }
+    } guard(c);
     for (int i = 0; i < nseg; ++i) {
         const int len = (i + 1 < nseg) ? every : nsteps - i * every;
         HIP_TRY(hipMemcpyAsync(c->ckpt[i], c->Un, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));

@@ -2,6 +2,7 @@
 // RK4 / forward-Euler stepping through captured HIP graphs, error reporting.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -199,7 +200,7 @@ int vti::run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0,
     int rc = vti::ensure_err_slots(&c->err_host, &c->err_host_cap, reps.size() + 1);
     if (rc) return rc;
     std::vector<hipGraphExec_t> ex(reps.size(), nullptr);
-    for (size_t r = 0; r < reps.size(); ++r)
+    for (size_t r = 0; r < reps.size() && !c->small_on; ++r)
         if (!(method == 0 && reps[r] == 1)) {
             rc = get_graph(c, method, dt, reps[r], &ex[r]);
             if (rc) return rc;
@@ -207,7 +208,22 @@ int vti::run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0,
     if (ev0) HIP_TRY(hipEventRecord(ev0, c->stream));
     for (size_t r = 0; r < reps.size(); ++r) {
         HIP_TRY(hipMemcpyAsync(c->err, c->err_init, sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
-        if (ex[r]) {
+        if (c->small_on) {  // the whole chunk in one launch, state in LDS
+            vt::SmallParams sp{};
+            sp.Un = c->Un;
+            sp.vtx = c->vtx;
+            sp.tab = c->tab;
+            sp.faces = c->faces;
+            sp.err = c->err;
+            sp.dt = dt;
+            sp.ne = c->ne;
+            sp.ld = c->ld;
+            sp.nfaces = c->nfaces;
+            sp.nsteps = reps[r];
+            sp.method = method;
+            for (int g = 0; g < 4; ++g) sp.fbeg[g] = c->fbeg[g];
+            HIP_TRY(vt::launch_small(sp, c->p, c->nq, c->stream));
+        } else if (ex[r]) {
             HIP_TRY(hipGraphLaunch(ex[r], c->stream));
         } else {  // single Euler step: Un -> Ua, then copy back
             HIP_TRY(enqueue_euler_step(c, dt, c->Un, c->Ua, 0));
@@ -319,6 +335,36 @@ int create_ctx(int n, const double* vertices, const int* neighbors, const int* t
     c->pairs = n % 2 == 0 ? 1 : 0;
     for (int e = 0; e < n && c->pairs; ++e)
         if (ns[3 * static_cast<size_t>(e) + 1] != (e ^ 1)) c->pairs = 0;
+    // Small-mesh path: the mesh fits one workgroup's LDS and every side's
+    // neighbour faces back across the same side, so each shared side can be
+    // listed once (as e | nb << 10 | s << 20 with e < nb).
+    std::vector<int> faces;
+    if (n <= vt::kSmallMaxElements && c->nq >= 1) {
+        bool ok = true;
+        for (int s = 0; s < 3 && ok; ++s) {  // listed by side index
+            c->fbeg[s] = static_cast<int>(faces.size());
+            for (int e = 0; e < n && ok; ++e) {
+                const int nb = ns[3 * static_cast<size_t>(e) + s];
+                if (nb == e || ns[3 * static_cast<size_t>(nb) + s] != e) ok = false;
+                else if (e < nb) faces.push_back(e | (nb << 10) | (s << 20));
+            }
+        }
+        c->fbeg[3] = static_cast<int>(faces.size());
+        int lim = 0, optin = 0;
+        if (!hip_ok(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device), "attribute") ||
+            !hip_ok(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, c->device), "attribute"))
+            return fail(VT_ERR_HIP);
+        c->small_ok = ok && static_cast<int>(faces.size()) * 2 == 3 * n &&
+                      vt::small_lds_bytes(p, c->nq, n) <= static_cast<size_t>(std::max(lim, optin));
+        if (!c->small_ok) faces.clear();
+    }
+    if (c->small_ok) {
+        if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&c->faces), sizeof(int) * faces.size()), "hipMalloc faces") ||
+            !hip_ok(hipMemcpy(c->faces, faces.data(), sizeof(int) * faces.size(), hipMemcpyHostToDevice), "upload faces"))
+            return fail(VT_ERR_HIP);
+        c->nfaces = static_cast<int>(faces.size());
+        c->small_on = true;
+    }
     for (int e = 0; e < n; ++e) {
         for (int k = 0; k < 6; ++k) hv[static_cast<size_t>(k) * c->ld + e] = vertices[6 * static_cast<size_t>(e) + k];
         for (int s = 0; s < 3; ++s) hn[static_cast<size_t>(s) * c->ld + e] = ns[3 * static_cast<size_t>(e) + s];
@@ -417,7 +463,8 @@ void vt_destroy(vt_ctx* c) {
     for (void* b : {static_cast<void*>(c->Un), static_cast<void*>(c->Ua), static_cast<void*>(c->Ub),
                     static_cast<void*>(c->acc), static_cast<void*>(c->vtx), static_cast<void*>(c->nbr),
                     static_cast<void*>(c->tab), static_cast<void*>(c->err), static_cast<void*>(c->l2ref),
-                    static_cast<void*>(c->l2tab), static_cast<void*>(c->l2part), static_cast<void*>(c->vrule)})
+                    static_cast<void*>(c->l2tab), static_cast<void*>(c->l2part), static_cast<void*>(c->vrule),
+                    static_cast<void*>(c->faces)})
         if (b) hipFree(b);
     if (c->err_init) hipHostFree(c->err_init);
     if (c->err_host) hipHostFree(c->err_host);
@@ -447,6 +494,15 @@ int vt_init_vortex_device(vt_ctx* c, int init_type, double epsilon, double x0, d
     HIP_TRY(hipStreamSynchronize(c->stream));
     return VT_OK;
 }
+
+int vt_set_small_path(vt_ctx* c, int enable) {
+    if (!c) return VT_ERR_ARG;
+    if (enable && !c->small_ok) return VT_ERR_MESH;
+    c->small_on = enable != 0;
+    return VT_OK;
+}
+
+int vt_get_small_path(const vt_ctx* c) { return c && c->small_on ? 1 : 0; }
 
 int vt_set_target_vortex(vt_ctx* c, double t, double epsilon, double x0, double y0) {
     if (!c) return VT_ERR_ARG;

@@ -49,6 +49,11 @@ struct vt_ctx {
     // of the least-squares type (uploaded on first use), and the objective's
     // target vortex {t, eps, x0, y0} set by vt_set_target_vortex.
     double* vrule = nullptr;
+    // Small-mesh path (vt_small.hpp): each shared side once, and whether
+    // vt_step runs through it (small_ok: the mesh qualifies).
+    int* faces = nullptr;
+    int nfaces = 0, fbeg[4] = {0, 0, 0, 0};
+    bool small_ok = false, small_on = false;
     bool tgt_vortex = false;
     double tgt_par[4] = {0.0, 0.0, 0.0, 0.0};
 };

@@ -145,8 +145,31 @@ struct VortexParams {
     int ne, ld, type, nqa, density_only;
 };
 
+// Small-mesh path (vt_small.hpp): one workgroup of kSmallThreads threads
+// runs nsteps steps of a whole mesh of at most kSmallMaxElements elements
+// with the stage state in LDS.  faces lists each shared side once, as
+// e | nb << 10 | s << 20 with e < nb the two elements and s the side (the
+// same side index on both), ordered by s: faces[fbeg[s] .. fbeg[s + 1]).
+// Un holds the state before and after the launch; the first failure goes to
+// err with stage ids counted from 0 in the launch.
+constexpr int kSmallThreads = 512;
+constexpr int kSmallMaxElements = kSmallThreads / 4;  // one thread per (variable, element)
+
+struct SmallParams {
+    double* Un;           // state planes [4*NP][ld]
+    const double* vtx;    // vertex planes
+    const double* tab;    // packed operator table
+    const int* faces;     // [nfaces]
+    unsigned long long* err;
+    double dt;
+    int ne, ld, nfaces, nsteps, method;  // method: 1 RK4, 0 forward Euler
+    int fbeg[4];
+};
+
 hipError_t launch_stage(const StageParams& prm, int p, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st);
+hipError_t launch_small(const SmallParams& prm, int p, int nq, hipStream_t st);
+size_t small_lds_bytes(int p, int nq, int ne);
 hipError_t launch_batched(const BatchParams& bp, int nq, const LaunchConfig& cfg, hipStream_t st);
 hipError_t launch_flux(const double* in, double* out, int* codes, int n, hipStream_t st);
 hipError_t launch_objective(const ObjParams& prm, int p, hipStream_t st);

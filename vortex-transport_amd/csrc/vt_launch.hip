@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "vt_stage.hpp"
+#include "vt_small.hpp"
 
 namespace vt {
 
@@ -278,6 +279,19 @@ hipError_t launch_batched(const BatchParams& bp, int nq, const LaunchConfig& cfg
 #undef X
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_small(const SmallParams& prm, int p, int nq, hipStream_t st) {
+    switch (nq) {
+#define X(n) case n: return NqLaunch<n>::small(prm, p, st);
+        VT_NQ_CASES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
+size_t small_lds_bytes(int p, int nq, int ne) {
+    return sizeof(double) * static_cast<size_t>(small_layout((p + 1) * (p + 2) / 2, p + 1, nq, ne).total);
 }
 
 hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st) {

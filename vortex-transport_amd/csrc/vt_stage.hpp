@@ -566,6 +566,7 @@ struct NqLaunch {
     static hipError_t stage(const StageParams& prm, int p, const LaunchConfig& cfg, hipStream_t st);
     static hipError_t batched(const BatchParams& bp, const LaunchConfig& cfg, hipStream_t st);
     static hipError_t adjoint(const AdjParams& prm, int p, hipStream_t st);
+    static hipError_t small(const SmallParams& prm, int p, hipStream_t st);
 };
 
 template <int NQ, int BS, int P>
