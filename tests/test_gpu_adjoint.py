@@ -213,3 +213,26 @@ def test_adjoint_256_directional_derivative(vt, gpu):
     fd = (_objective_run(vt, s, U0 + h * d, dt, nsteps, tgt) - _objective_run(vt, s, U0 - h * d, dt, nsteps, tgt)) / (2 * h)
     ad = np.vdot(lam0, d)
     assert abs(fd - ad) < 1e-6 * abs(ad), (fd, ad)
+
+
+def test_adjoint_256_parameter_gradient(vt, gpu):
+    """dJ/d(epsilon, x0, y0) of the adjoint workload through the initial
+    state's closed-form tangent, against central differences (h = 1e-6) of
+    the full 1000-step forward runs; relative 1e-6 (the differences' own
+    truncation and rounding at this h are ~1e-8)."""
+    dt, nsteps = 1e-4, 1000
+    mesh = vt.generate_mesh(nx=256, ny=256)
+    s = vt.Solver(mesh, 3, 8)
+    tgt = s.exact_vortex(dt * nsteps, 0.32, 0.1, -0.05)[:, :, 0]
+    s.initialize_hydrodinamics(0)
+    J, lam0 = s.adjoint(dt, nsteps, tgt, every=32)
+    T = s.vortex_tangent(0.0, 0.3, 0.0, 0.0)
+    gp = np.tensordot(T, lam0, axes=([1, 2, 3], [0, 1, 2]))
+    h = 1e-6
+    for i, (e, x, y) in enumerate([(h, 0, 0), (0, h, 0), (0, 0, h)]):
+        Jp = _objective_run(vt, s, s.exact_vortex(0.0, 0.3 + e, x, y), dt, nsteps, tgt)
+        Jm = _objective_run(vt, s, s.exact_vortex(0.0, 0.3 - e, -x, -y), dt, nsteps, tgt)
+        fd = (Jp - Jm) / (2 * h)
+        print("parameter %d: adjoint %.9e, central difference %.9e" % (i, gp[i], fd), flush=True)
+        assert abs(fd - gp[i]) < 1e-6 * abs(gp[i]) + 1e-14, (i, fd, gp[i])
+    s.close()
