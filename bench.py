@@ -270,6 +270,13 @@ def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
         t0 = time.perf_counter()
         J, lam0 = s.adjoint(dt, steps, tgt, every)
         tas.append(time.perf_counter() - t0)
+    # The same adjoint with every stage state of the run kept in HBM (one
+    # segment of 4 x steps states, 168 GB at 256^2 x 1000 steps): no recompute.
+    s.set_state(U0)
+    t0 = time.perf_counter()
+    Js, lam_s = s.adjoint(dt, steps, tgt, steps)
+    t_stored = time.perf_counter() - t0
+    stored_bytes = 4 * steps * s.ne * 10 * 4 * 8
     s.close()
     tf, ta = float(np.median(tfs)), float(np.median(tas))
     return {"workload": "%dx%d cells, 10x10 domain, p=3, integration_order 8, %d RK4 steps of dt %g, "
@@ -277,7 +284,11 @@ def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
                         "(eps 0.32, centre (0.1, -0.05))" % (n, n, steps, dt, every),
             "elements": 2 * n * n, "forward_s": tf, "forward_plus_adjoint_s": ta, "ratio": ta / tf,
             "runs": {"forward_s": tfs, "forward_plus_adjoint_s": tas}, "J": J,
-            "grad_norm": float(np.linalg.norm(lam0))}
+            "grad_norm": float(np.linalg.norm(lam0)),
+            "all_stages_stored": {"every": steps, "stage_states_gb": stored_bytes / 1e9,
+                                  "forward_plus_adjoint_s": t_stored, "ratio": t_stored / tf,
+                                  "rel_l2_vs_checkpointed": float(np.linalg.norm(lam_s - lam0) / np.linalg.norm(lam0)),
+                                  "J_equal": bool(Js == J)}}
 
 
 def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):

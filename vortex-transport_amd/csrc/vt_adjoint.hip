@@ -116,8 +116,10 @@ int download_planes(vt_ctx* c, const double* dev, double* U) { return vti::downl
 
 // Forward stage kernels of one step from slot S0 into slots S1, S2, S3 and
 // next; with next null the fourth stage (whose output the reverse sweep of the
-// step does not read) is skipped.
-hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* S2, double* S3, double* next) {
+// step does not read) is skipped.  Failures carry the stage ids
+// stage_base + 0..3.
+hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* S2, double* S3, double* next,
+                          int stage_base = 0) {
     vt::StageParams prm{};
     prm.vtx = c->vtx;
     prm.nbr = c->nbr;
@@ -136,11 +138,24 @@ hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* 
         prm.Uout = out[s];
         prm.Un = (s == 1 || s == 2) ? S0 : nullptr;
         prm.mode = vt::kModeRk1 + s;
-        prm.stage_id = s;
+        prm.stage_id = stage_base + s;
         hipError_t e = vt::launch_stage(prm, c->p, c->nq, c->cfg, c->stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// The stage states of one segment: 4 * min(nsteps, every) slots.
+int ensure_seg(vt_ctx* c, int nsteps, int every) {
+    const int need = 4 * (nsteps < every ? nsteps : every);
+    if (c->seg_slots >= need) return VT_OK;
+    c->rec_last_stored = false;
+    if (c->seg) (void)hipFree(c->seg);
+    c->seg = nullptr;
+    c->seg_slots = 0;
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->seg), sizeof(double) * slot_elems(c) * need));
+    c->seg_slots = need;
+    return VT_OK;
 }
 
 }  // namespace
@@ -149,14 +164,8 @@ int vti::backward_sweep(vt_ctx* c) {
     const int every = c->rec_every, nsteps = c->rec_steps, nseg = (nsteps + every - 1) / every;
     const double dt = c->rec_dt;
     const size_t n = slot_elems(c);
-    const int need = 4 * (nsteps < every ? nsteps : every);
-    if (c->seg_slots < need) {
-        if (c->seg) (void)hipFree(c->seg);
-        c->seg = nullptr;
-        c->seg_slots = 0;
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->seg), sizeof(double) * n * need));
-        c->seg_slots = need;
-    }
+    int rc = ensure_seg(c, nsteps, every);
+    if (rc) return rc;
     double *lam = c->adj, *acc = c->adj + n, *y[2] = {c->adj + 2 * n, c->adj + 3 * n};
     vt::AdjParams a = adj_params(c, vt::kAdjSeed);
     a.lam = lam;
@@ -172,9 +181,14 @@ int vti::backward_sweep(vt_ctx* c) {
     for (int i = nseg - 1; i >= 0; --i) {
         const int len = (i + 1 < nseg) ? every : nsteps - i * every;
         seg_i = i;
-        for (int s = 0; s < len; ++s)
-            HIP_TRY(recompute_step(c, dt, slot(s, 0), slot(s, 1), slot(s, 2), slot(s, 3),
-                                   s + 1 < len ? slot(s + 1, 0) : nullptr));
+        // The last segment's stage states were kept by the recording itself
+        // (until a recompute of an earlier segment overwrites them).
+        if (!(i == nseg - 1 && c->rec_last_stored)) {
+            c->rec_last_stored = false;
+            for (int s = 0; s < len; ++s)
+                HIP_TRY(recompute_step(c, dt, slot(s, 0), slot(s, 1), slot(s, 2), slot(s, 3),
+                                       s + 1 < len ? slot(s + 1, 0) : nullptr));
+        }
         for (int s = len - 1; s >= 0; --s)
             for (int st = 3; st >= 0; --st) {
                 vt::AdjParams b = adj_params(c, vt::kAdj1 + st);
@@ -205,6 +219,7 @@ void vti::free_adjoint(vt_ctx* c) {
     c->part_cap = 0;
     c->seg_slots = 0;
     c->recorded = false;
+    c->rec_last_stored = false;
 }
 
 extern "C" {
@@ -252,14 +267,40 @@ int vt_adjoint_forward(vt_ctx* c, double dt, int nsteps, int every) {
         explicit StageKernels(vt_ctx* x) : c(x), was(x->small_on) { c->small_on = false; }
         ~StageKernels() { c->small_on = was; }
     } guard(c);
+    c->rec_last_stored = false;
+    int rc = ensure_seg(c, nsteps, every);
+    if (rc) return rc;
     for (int i = 0; i < nseg; ++i) {
         const int len = (i + 1 < nseg) ? every : nsteps - i * every;
         HIP_TRY(hipMemcpyAsync(c->ckpt[i], c->Un, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-        int rc = vti::run_steps(c, 1, dt, len, nullptr, nullptr);
-        if (rc) {
-            if (rc == VT_ERR_NONPHYSICAL) c->last_err[4] += i * every;
-            return rc;
+        if (i + 1 < nseg) {
+            rc = vti::run_steps(c, 1, dt, len, nullptr, nullptr);
+            if (rc) {
+                if (rc == VT_ERR_NONPHYSICAL) c->last_err[4] += i * every;
+                return rc;
+            }
+            continue;
         }
+        // The last segment, which the reverse sweep takes first, runs with its
+        // stage states written straight into the segment slots, so the sweep
+        // need not recompute it (with every >= nsteps: the whole run).
+        auto slot = [&](int step, int stage) {
+            return (step == 0 && stage == 0) ? c->ckpt[i] : c->seg + (static_cast<size_t>(4) * step + stage) * n;
+        };
+        rc = vti::ensure_err_slots(&c->err_host, &c->err_host_cap, 1);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(c->err, c->err_init, sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+        for (int s = 0; s < len; ++s)
+            HIP_TRY(recompute_step(c, dt, slot(s, 0), slot(s, 1), slot(s, 2), slot(s, 3),
+                                   s + 1 < len ? slot(s + 1, 0) : c->Un, 4 * s));
+        HIP_TRY(hipMemcpyAsync(c->err_host, c->err, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->err_host[0] != vt::kNoError) {
+            vti::decode_error(c, c->err_host[0], i * every, 4);
+            return VT_ERR_NONPHYSICAL;
+        }
+        for (int s = 0; s < len; ++s) c->time += dt;
+        c->rec_last_stored = true;
     }
     c->rec_steps = nsteps;
     c->rec_every = every;

@@ -180,7 +180,9 @@ int get_graph(vt_ctx* c, int method, double dt, int k, hipGraphExec_t* out) {
 
 int read_planes(vt_ctx* c, const double* dev, double* U) { return vti::download_state(c, dev, U); }
 
-void decode_error(vt_ctx* c, unsigned long long key, int step_base, int per_step) {
+}  // namespace
+
+void vti::decode_error(vt_ctx* c, unsigned long long key, int step_base, int per_step) {
     const unsigned long long stage = key >> 40;
     unsigned long long t = key & ((1ull << 40) - 1);
     c->last_err[3] = static_cast<int>(t % 4); t /= 4;
@@ -190,8 +192,6 @@ void decode_error(vt_ctx* c, unsigned long long key, int step_base, int per_step
     c->last_err[4] = step_base + static_cast<int>(stage / per_step) + 1;
     c->last_err[5] = static_cast<int>(stage % per_step) + 1;
 }
-
-}  // namespace
 
 int vti::run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0, hipEvent_t ev1) {
     if (method != 0 && method != 1) return VT_ERR_METHOD;
@@ -237,7 +237,7 @@ int vti::run_steps(vt_ctx* c, int method, double dt, int nsteps, hipEvent_t ev0,
     for (size_t r = 0; r < reps.size(); ++r) {
         // Element::time advances by dt once per step (Timestepping.cpp:91,315).
         if (c->err_host[r] != vt::kNoError) {
-            decode_error(c, c->err_host[r], base, method == 1 ? 4 : 1);
+            vti::decode_error(c, c->err_host[r], base, method == 1 ? 4 : 1);
             return VT_ERR_NONPHYSICAL;
         }
         for (int i = 0; i < reps[r]; ++i) c->time += dt;
@@ -600,7 +600,7 @@ int vt_rhs(vt_ctx* c, double* dUdt) {
     unsigned long long key = 0;
     HIP_TRY(hipMemcpy(&key, c->err, sizeof(key), hipMemcpyDeviceToHost));
     if (key != vt::kNoError) {
-        decode_error(c, key, -1, 1);
+        vti::decode_error(c, key, -1, 1);
         return VT_ERR_NONPHYSICAL;
     }
     return VT_OK;
@@ -673,7 +673,7 @@ int vt_stage_kernel_times(vt_ctx* c, double dt, int nsteps, double* ms) {
     unsigned long long key = 0;
     HIP_TRY(hipMemcpy(&key, c->err, sizeof(key), hipMemcpyDeviceToHost));
     if (key != vt::kNoError) {
-        decode_error(c, key, 0, 4);
+        vti::decode_error(c, key, 0, 4);
         return VT_ERR_NONPHYSICAL;
     }
     return VT_OK;

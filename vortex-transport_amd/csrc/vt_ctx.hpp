@@ -44,6 +44,7 @@ struct vt_ctx {
     int rec_steps = 0, rec_every = 0;
     double rec_dt = 0.0;
     bool recorded = false;
+    bool rec_last_stored = false;            // seg holds the last segment's stage states from the recording
     unsigned long long rec_id = 0;           // number of vt_adjoint_forward calls so far
     // Device set-up of the vortex state: area rule, basis values and Minv_ref
     // of the least-squares type (uploaded on first use), and the objective's
@@ -125,6 +126,10 @@ int vortex_device(vt_ctx* c, int nr, const int* e0, const int* n, const double* 
 // The reverse sweep over the recorded forward run, from lam = dJ/dU_T in the
 // lam planes to lam = dJ/dU_0 in the same planes.
 int backward_sweep(vt_ctx* c);
+
+// Fills last_err from an error key: element, point, side, code, the step
+// (step_base + the key's stage / per_step + 1) and the stage within it.
+void decode_error(vt_ctx* c, unsigned long long key, int step_base, int per_step);
 
 // Pinned host error slots, at least n of them.
 int ensure_err_slots(unsigned long long** slots, size_t* cap, size_t n);
