@@ -291,7 +291,7 @@ hipError_t launch_small(const SmallParams& prm, int p, int nq, hipStream_t st) {
 }
 
 size_t small_lds_bytes(int p, int nq, int ne) {
-    return sizeof(double) * static_cast<size_t>(small_layout((p + 1) * (p + 2) / 2, p + 1, nq, ne).total);
+    return sizeof(double) * static_cast<size_t>(small_layout((p + 1) * (p + 2) / 2, p + 1, nq, small_pad(ne)).total);
 }
 
 hipError_t launch_adjoint(const AdjParams& prm, int p, int nq, hipStream_t st) {

@@ -38,9 +38,10 @@ namespace vt {
 struct SmallLayout {
     int nep, ns, nfp, S, Fs, Nf, Tw, Fc, key, total;
 };
-__host__ __device__ inline SmallLayout small_layout(int np, int ne1, int nq, int ne) {
+__host__ __device__ constexpr int small_pad(int ne) { return (ne + 63) / 64 * 64; }
+__host__ __device__ inline SmallLayout small_layout(int np, int ne1, int nq, int nep) {
     SmallLayout L;
-    L.nep = (ne + 63) / 64 * 64;
+    L.nep = nep;
     L.ns = L.nep + 1;
     L.nfp = 3 * L.nep / 2;
     L.S = 0;
@@ -56,9 +57,9 @@ __host__ __device__ inline SmallLayout small_layout(int np, int ne1, int nq, int
 // Phase A item: side SD of the listed side f (elements e < nb), Gauss point q
 // in e's orientation.  Traces from the stage state, one Roe flux, stored for e
 // at q and, negated, for nb at NQ - 1 - q; failures of either element go to key.
-template <int P, int NQ, int SD>
+template <int P, int NQ, int SD, int ns>
 __device__ __forceinline__ void small_side_flux(const double* S, double* Fs, const double* Nf, const double* Tw,
-                                                const int* Fc, unsigned long long* key, int ns, int nfp, int f, int q,
+                                                const int* Fc, unsigned long long* key, int nfp, int f, int q,
                                                 int st) {
     constexpr int NP = (P + 1) * (P + 2) / 2;
     constexpr int NE = P + 1;
@@ -95,8 +96,8 @@ __device__ __forceinline__ void small_side_flux(const double* S, double* Fs, con
 
 // Volume term of variable K at element e: r_i += Sxi_ij G^xi_j + Seta_ij G^eta_j
 // with G = adj(J) F_K(U_j), the nodal flux as the stage kernel forms it.
-template <int P, int K>
-__device__ __forceinline__ void small_volume(const double* S, int ns, int e, double a00, double a01, double a10,
+template <int P, int K, int ns>
+__device__ __forceinline__ void small_volume(const double* S, int e, double a00, double a01, double a10,
                                              double a11, ctab_t T, double (&r)[(P + 1) * (P + 2) / 2]) {
     constexpr int NP = (P + 1) * (P + 2) / 2;
     constexpr double gmi = 1.4 - 1.0;
@@ -130,13 +131,15 @@ __device__ __forceinline__ double rounded_product(double a, double b) {
     return p;
 }
 
-template <int P, int NQ>
+template <int P, int NQ, int NEP>
 __global__ void __launch_bounds__(kSmallThreads, 1) small_run_kernel(SmallParams prm) {
     constexpr int NP = (P + 1) * (P + 2) / 2;
     constexpr int NE = P + 1;
     extern __shared__ double lds[];
-    const SmallLayout L = small_layout(NP, NE, NQ, prm.ne);
-    const int nep = L.nep, ns = L.ns;
+    // nep = ne rounded up to 64 (NEP, a launch parameter), so that every LDS
+    // row offset is a compile-time constant
+    constexpr int nep = NEP, ns = NEP + 1;
+    const SmallLayout L = small_layout(NP, NE, NQ, NEP);
     // first padded phase-A item of each side group
     int ib[4];
     ib[0] = 0;
@@ -212,9 +215,9 @@ __global__ void __launch_bounds__(kSmallThreads, 1) small_run_kernel(SmallParams
             const int li = it - ib[sd];
             if (li >= (prm.fbeg[sd + 1] - prm.fbeg[sd]) * NQ) continue;
             const int f = prm.fbeg[sd] + li / NQ, q = li - (li / NQ) * NQ;
-            if (sd == 0) small_side_flux<P, NQ, 0>(S, Fs, Nf, Tw, Fc, key, ns, L.nfp, f, q, st);
-            else if (sd == 1) small_side_flux<P, NQ, 1>(S, Fs, Nf, Tw, Fc, key, ns, L.nfp, f, q, st);
-            else small_side_flux<P, NQ, 2>(S, Fs, Nf, Tw, Fc, key, ns, L.nfp, f, q, st);
+            if (sd == 0) small_side_flux<P, NQ, 0, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, f, q, st);
+            else if (sd == 1) small_side_flux<P, NQ, 1, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, f, q, st);
+            else small_side_flux<P, NQ, 2, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, f, q, st);
         }
         // The volume term reads only the stage state: formed before the
         // barrier, beside the flux evaluations.
@@ -223,10 +226,10 @@ __global__ void __launch_bounds__(kSmallThreads, 1) small_run_kernel(SmallParams
         for (int i = 0; i < NP; ++i) r[i] = 0.0;
         if (bact) {
             switch (kb) {
-            case 0: small_volume<P, 0>(S, ns, eb, a00, a01, a10, a11, T, r); break;
-            case 1: small_volume<P, 1>(S, ns, eb, a00, a01, a10, a11, T, r); break;
-            case 2: small_volume<P, 2>(S, ns, eb, a00, a01, a10, a11, T, r); break;
-            default: small_volume<P, 3>(S, ns, eb, a00, a01, a10, a11, T, r); break;
+            case 0: small_volume<P, 0, ns>(S, eb, a00, a01, a10, a11, T, r); break;
+            case 1: small_volume<P, 1, ns>(S, eb, a00, a01, a10, a11, T, r); break;
+            case 2: small_volume<P, 2, ns>(S, eb, a00, a01, a10, a11, T, r); break;
+            default: small_volume<P, 3, ns>(S, eb, a00, a01, a10, a11, T, r); break;
             }
         }
         __syncthreads();
@@ -289,14 +292,21 @@ __global__ void __launch_bounds__(kSmallThreads, 1) small_run_kernel(SmallParams
     if (tid == 0 && *key != kNoError) atomicMin(prm.err, *key);
 }
 
-template <int NQ, int P>
-hipError_t launch_small_p(const SmallParams& prm, hipStream_t st) {
-    const size_t lds = sizeof(double) * small_layout((P + 1) * (P + 2) / 2, P + 1, NQ, prm.ne).total;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&small_run_kernel<P, NQ>),
+template <int NQ, int P, int NEP>
+hipError_t launch_small_pn(const SmallParams& prm, hipStream_t st) {
+    const size_t lds = sizeof(double) * small_layout((P + 1) * (P + 2) / 2, P + 1, NQ, NEP).total;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&small_run_kernel<P, NQ, NEP>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((small_run_kernel<P, NQ>), dim3(1), dim3(kSmallThreads), lds, st, prm);
+    hipLaunchKernelGGL((small_run_kernel<P, NQ, NEP>), dim3(1), dim3(kSmallThreads), lds, st, prm);
     return hipGetLastError();
+}
+
+template <int NQ, int P>
+hipError_t launch_small_p(const SmallParams& prm, hipStream_t st) {
+    if (prm.ne <= 64) return launch_small_pn<NQ, P, 64>(prm, st);
+    if (prm.ne <= 128) return launch_small_pn<NQ, P, 128>(prm, st);
+    return hipErrorInvalidValue;
 }
 
 template <int NQ>
