@@ -149,8 +149,10 @@ def trace_avg_launch_ms(n_elem, block):
 
 def batched_figure(vt, steps, dt=1e-4):
     """Ensemble of 64 instances (seed 3: cells per side 32..256, p 1..3,
-    epsilon 0.1..0.4, centre in [-2.5, 2.5]^2 of a 10 x 10 domain), one batched
-    launch per RK stage with the autotuned launch shape, against the same 64
+    epsilon 0.1..0.4, centre in [-2.5, 2.5]^2 of a 10 x 10 domain), every
+    instance's RK stage in one graph with the autotuned launch shape (one
+    launch for all groups, or one per (p, order) group run concurrently),
+    against the same 64
     instances stepped one plain solver after another; medians of three runs
     of each."""
     import torch

@@ -242,7 +242,8 @@ int vt_batch_create(int n_inst, const int *n_elem, const int *p, const int *orde
                     const int *neighbors, const int *type, vt_batch **out);
 void vt_batch_destroy(vt_batch *b);
 int vt_batch_num_instances(const vt_batch *b);
-/* Kernel launches per RK stage (distinct quadrature sizes, at most 4 orders p each). */
+/* Kernel launches per RK stage: distinct quadrature sizes (at most 4 orders p
+ * each), or (p, order) groups in the concurrent launch mode. */
 int vt_batch_num_launches(const vt_batch *b);
 /* Vortex initial state per instance (vt_init_vortex with epsilon[i], x0[i], y0[i]). */
 int vt_batch_init_vortex(vt_batch *b, int init_type, const double *epsilon, const double *x0, const double *y0);
@@ -266,6 +267,11 @@ int vt_batch_prepare_steps(vt_batch *b, int method, double dt, int nsteps);
 int vt_batch_step_timed(vt_batch *b, int method, double dt, int nsteps, double *ms);
 /* instance, element (local), quadrature point, side, code, step, stage. */
 int vt_batch_last_error(const vt_batch *b, int *info /* [7] */);
+/* Launch shape of the ensemble's stage launches: block 64, 128 or 256;
+ * persistent 0: one launch per quadrature size, one workgroup per tile;
+ * 1: the same launch with 8 * slots_per_xcd resident workgroups;
+ * 2: one launch per (p, order) group, the groups run concurrently on streams
+ * of their own (slots_per_xcd ignored). */
 int vt_batch_set_launch(vt_batch *b, int block, int persistent, int slots_per_xcd);
 int vt_batch_get_launch(const vt_batch *b, int *cfg /* [3] */);
 /* Times trial_steps (1..64) RK4 steps of dt for each candidate launch shape,

@@ -92,13 +92,17 @@ def test_batch_instances_match_oracle(vt, oracle, gpu):
         assert rel(b.state(i), o.state()) < 1e-12
 
 
-@pytest.mark.parametrize("block,persistent,slots", [(256, False, 0), (128, False, 0), (64, False, 0), (256, True, 1),
-                                                     (128, True, 3), (64, True, 7)])
-def test_batch_launch_shapes_agree(vt, gpu, block, persistent, slots):
+@pytest.mark.parametrize("block,persistent,slots,concurrent", [(256, False, 0, False), (128, False, 0, False),
+                                                                (64, False, 0, False), (256, True, 1, False),
+                                                                (128, True, 3, False), (64, True, 7, False),
+                                                                (64, False, 0, True), (256, False, 0, True)])
+def test_batch_launch_shapes_agree(vt, gpu, block, persistent, slots, concurrent):
     dt, steps = 1e-3, 9
     meshes, b = _build(vt, SMALL)
-    b.set_launch(block, persistent, slots)
-    assert b.launch() == {"block": block, "persistent": persistent, "slots_per_xcd": slots}
+    b.set_launch(block, persistent, slots, concurrent=concurrent)
+    assert b.launch() == {"block": block, "persistent": persistent, "concurrent": concurrent, "slots_per_xcd": slots}
+    if concurrent:
+        assert b.num_launches == len({(c[1], c[2]) for c in SMALL})
     b.rk4(dt, steps)
     for i, (m, c) in enumerate(zip(meshes, SMALL)):
         assert same(b.state(i), _single(vt, m, c, 1, dt, steps)), i

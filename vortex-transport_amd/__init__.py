@@ -509,6 +509,11 @@ class Solver:
 # counterpart; each instance is what Solver builds from its own mesh)
 # ---------------------------------------------------------------------------
 
+def _launch_dict(c):
+    return {"block": int(c[0]), "persistent": int(c[1]) == 1, "concurrent": int(c[1]) == 2,
+            "slots_per_xcd": int(c[2])}
+
+
 class Batch:
     """Device-resident ensemble of independent vortex problems.
 
@@ -616,13 +621,16 @@ class Batch:
         _check(lib().vt_batch_step_timed(self._h, method, dt, nsteps, ctypes.byref(ms)), self)
         return ms.value
 
-    def set_launch(self, block=64, persistent=False, slots_per_xcd=0):
-        _check(lib().vt_batch_set_launch(self._h, block, int(bool(persistent)), slots_per_xcd))
+    def set_launch(self, block=64, persistent=False, slots_per_xcd=0, concurrent=False):
+        """concurrent: one launch per (p, order) group and stage, the groups
+        on streams of their own, instead of one launch per quadrature size."""
+        mode = 2 if concurrent else int(bool(persistent))
+        _check(lib().vt_batch_set_launch(self._h, block, mode, slots_per_xcd))
 
     def launch(self):
         c = np.zeros(3, dtype=np.int32)
         _check(lib().vt_batch_get_launch(self._h, _i(c)))
-        return {"block": int(c[0]), "persistent": bool(c[1]), "slots_per_xcd": int(c[2])}
+        return _launch_dict(c)
 
     def autotune(self, dt, trial_steps=4):
         """Picks the fastest launch shape over trial RK4 steps; state and
@@ -630,7 +638,7 @@ class Batch:
         c = np.zeros(3, dtype=np.int32)
         ms = ctypes.c_double()
         _check(lib().vt_batch_autotune(self._h, dt, trial_steps, _i(c), ctypes.byref(ms)), self)
-        return {"block": int(c[0]), "persistent": bool(c[1]), "slots_per_xcd": int(c[2])}, ms.value
+        return _launch_dict(c), ms.value
 
     # --- ensemble adjoint ---------------------------------------------------
 

@@ -38,6 +38,12 @@ struct vt_batch {
     double time = 0.0;
     int last_err[7] = {-1, -1, -1, -1, 0, 0, 0};
     std::vector<double> tgt_par;               // [n][4] target vortex {t, eps, x0, y0}, or empty
+    // Launch mode 2 (cfg.persistent == 2): each group's stage is its own
+    // single-mesh launch on the group's stream, the groups forked from and
+    // joined back to the batch stream, so they run concurrently, each kernel
+    // at its own occupancy.
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
 };
 
 namespace {
@@ -49,10 +55,43 @@ void drop_graphs(vt_batch* b) {
     b->graphs.clear();
 }
 
-// One batched launch per quadrature size for one stage.  bufs(c) gives the
-// stage's {Uin, Uout, Un} of group context c.
+// One batched launch per quadrature size for one stage (or, in launch mode 2,
+// one launch per group on concurrent streams).  bufs(c) gives the stage's
+// {Uin, Uout, Un} of group context c.
 template <class Bufs>
 hipError_t enqueue_stage(vt_batch* b, double dt, int mode, int stage_id, Bufs bufs) {
+    if (b->cfg.persistent == 2) {
+        hipError_t e = hipEventRecord(b->fork, b->stream);
+        vt::LaunchConfig one;
+        one.block = b->cfg.block;
+        for (size_t gi = 0; gi < b->g.size() && e == hipSuccess; ++gi) {
+            vt_ctx* c = b->g[gi];
+            const double* io[3];
+            bufs(c, io);
+            vt::StageParams prm{};
+            prm.Uin = io[0];
+            prm.Uout = const_cast<double*>(io[1]);
+            prm.Un = io[2];
+            prm.acc = c->acc;
+            prm.vtx = c->vtx;
+            prm.nbr = c->nbr;
+            prm.tab = c->tab;
+            prm.err = b->err;
+            prm.dt = dt;
+            prm.ne = c->ne;
+            prm.ld = c->ld;
+            prm.mode = mode;
+            prm.stage_id = stage_id;
+            prm.err_base = static_cast<int>(b->gbase[gi]);
+            prm.pairs = c->pairs;
+            prm.blocks_per_xcd = vt::tiles_per_xcd(c->ne, one.block);
+            e = hipStreamWaitEvent(c->stream, b->fork, 0);
+            if (e == hipSuccess) e = vt::launch_stage(prm, c->p, c->nq, one, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(b->join[gi], c->stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join[gi], 0);
+        }
+        return e;
+    }
     for (const auto& L : b->launches) {
         vt::BatchParams bp{};
         bp.ngroups = static_cast<int>(L.size());
@@ -387,6 +426,11 @@ int vt_batch_create(int n_inst, const int* n_elem, const int* p, const int* orde
         }
         b->launches.push_back({static_cast<int>(i)});
     }
+    b->join.assign(b->g.size(), nullptr);
+    if (!hip_ok(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate"))
+        return fail(VT_ERR_HIP);
+    for (hipEvent_t& ev : b->join)
+        if (!hip_ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate")) return fail(VT_ERR_HIP);
     if (!hip_ok(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate") ||
         !hip_ok(hipMalloc(reinterpret_cast<void**>(&b->err), sizeof(unsigned long long)), "hipMalloc err") ||
         !hip_ok(hipHostMalloc(reinterpret_cast<void**>(&b->err_init), sizeof(unsigned long long), hipHostMallocDefault),
@@ -404,12 +448,18 @@ void vt_batch_destroy(vt_batch* b) {
     if (b->err) hipFree(b->err);
     if (b->err_init) hipHostFree(b->err_init);
     if (b->err_host) hipHostFree(b->err_host);
+    if (b->fork) hipEventDestroy(b->fork);
+    for (hipEvent_t ev : b->join)
+        if (ev) hipEventDestroy(ev);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
 }
 
 int vt_batch_num_instances(const vt_batch* b) { return b ? b->n : 0; }
-int vt_batch_num_launches(const vt_batch* b) { return b ? static_cast<int>(b->launches.size()) : 0; }
+int vt_batch_num_launches(const vt_batch* b) {
+    if (!b) return 0;
+    return static_cast<int>(b->cfg.persistent == 2 ? b->g.size() : b->launches.size());
+}
 
 int vt_batch_init_vortex(vt_batch* b, int init_type, const double* eps, const double* x0, const double* y0) {
     if (!b || !eps || !x0 || !y0) return VT_ERR_ARG;
@@ -536,10 +586,11 @@ int vt_batch_last_error(const vt_batch* b, int* info) {
 }
 
 int vt_batch_set_launch(vt_batch* b, int block, int persistent, int slots_per_xcd) {
-    if (!b || (block != 64 && block != 128 && block != 256) || slots_per_xcd < 0 || (persistent && slots_per_xcd < 1))
+    if (!b || (block != 64 && block != 128 && block != 256) || slots_per_xcd < 0 || persistent < 0 || persistent > 2 ||
+        (persistent == 1 && slots_per_xcd < 1))
         return VT_ERR_ARG;
     b->cfg.block = block;
-    b->cfg.persistent = persistent ? 1 : 0;
+    b->cfg.persistent = persistent;
     b->cfg.slots_per_xcd = slots_per_xcd;
     drop_graphs(b);
     return VT_OK;
@@ -561,8 +612,8 @@ int vt_batch_autotune(vt_batch* b, double dt, int trial_steps, int* best_cfg, do
     struct Cand { int block, persistent, slots; };
     // Workgroups resident per CU at p = 3 (LDS 320 B and ~250 VGPRs per
     // element): 2 of 256 threads, 4 of 128, 8 of 64; 32 CUs per XCD.
-    const Cand cands[] = {{64, 0, 0},   {128, 0, 0},  {256, 0, 0},  {64, 1, 256},
-                          {128, 1, 128}, {256, 1, 64}, {128, 1, 64}, {256, 1, 32}};
+    const Cand cands[] = {{64, 0, 0},    {128, 0, 0},  {256, 0, 0}, {64, 1, 256}, {128, 1, 128},
+                          {256, 1, 64},  {128, 1, 64}, {256, 1, 32}, {64, 2, 0},   {128, 2, 0}};
     std::vector<double*> saved(b->g.size(), nullptr);
     auto cleanup = [&]() { for (double* s : saved) if (s) hipFree(s); };
     for (size_t i = 0; i < b->g.size(); ++i) {
