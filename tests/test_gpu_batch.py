@@ -233,16 +233,17 @@ def test_batch_groups_with_and_without_cell_pairs(vt, gpu):
 
 def test_ensemble_64_autotuned_equals_every_instance(vt, gpu):
     # The 64-instance ensemble at its full size (seed 3: 32..256 cells per
-    # side, p 1..3; 3,054,380 elements) in one launch per stage with the
-    # launch shape the autotuner picks (the test above uses the default
-    # shape and every seventh instance): every instance equals its plain run.
+    # side, p 1..3; 3,054,380 elements) with the launch shape the autotuner
+    # picks (one launch per stage, or one per (p, order) group concurrently;
+    # the test above uses the default shape and every seventh instance):
+    # every instance equals its plain run.
     draws = vt.ensemble_draws(3, 64)
     meshes = [vt.generate_mesh(nx=d["cells"], ny=d["cells"]) for d in draws]
     args = ([d["epsilon"] for d in draws], [d["x0"] for d in draws], [d["y0"] for d in draws])
     b = vt.Batch(meshes, [d["p"] for d in draws], 8)
     b.initialize_hydrodinamics(0, *args)
-    b.autotune(1e-4, 2)
-    assert b.num_launches == 1
+    cfg, _ = b.autotune(1e-4, 2)
+    assert b.num_launches == (3 if cfg["concurrent"] else 1)
     b.rk4(1e-4, 6)
     for i, (m, d) in enumerate(zip(meshes, draws)):
         U = _single(vt, m, (d["cells"], d["p"], 8, d["epsilon"], d["x0"], d["y0"]), 1, 1e-4, 6)
