@@ -100,6 +100,33 @@ def settle_clocks(s, dt, seconds):
     return n
 
 
+def stream_rates(n=1 << 28, reps=10):
+    """Practical HBM rates of plain streaming kernels on the same device, as a
+    yardstick beside the published 8 TB/s: torch's copy (read + write), triad
+    c = a + 2 b (two reads, one write), sum (read) and fill (write) over
+    2 GiB float64 arrays, HIP events around `reps` back-to-back launches."""
+    import torch
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.ones_like(a)
+    c = torch.empty_like(a)
+    out = {}
+    for name, f, nbytes in (("copy", lambda: c.copy_(a), 16 * n), ("triad", lambda: torch.add(a, b, alpha=2.0, out=c), 24 * n),
+                            ("read", lambda: a.sum(), 8 * n), ("write", lambda: c.fill_(1.0), 8 * n)):
+        for _ in range(3):
+            f()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name + "_GBps"] = nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b, c
+    torch.cuda.empty_cache()
+    return out
+
+
 def profile_pass(s, dt):
     import torch
     s.stage_kernel_times(dt, 2)
@@ -627,6 +654,11 @@ def main():
                  "executed_ops_per_launch": fp64_ops},
     }
     s.close()
+    st = stream_rates()
+    st["stage_kernel_algorithmic_vs_triad"] = achieved / st["triad_GBps"]
+    if traffic is not None:
+        st["stage_kernel_traffic_vs_triad"] = traffic / avg_launch_s / 1e9 / st["triad_GBps"]
+    line["hbm_stream"] = st
     if not a.no_batched:
         line["batched"] = batched_figure(vt, a.batched_steps)
     if a.orders:
