@@ -118,8 +118,10 @@ __device__ __forceinline__ void small_volume(const double* S, int e, double a00,
         const double gx = a00 * fx + a01 * fy;
         const double gy = a10 * fx + a11 * fy;
 #pragma unroll
-        for (int i = 0; i < NP; ++i)
-            r[i] += T[tab_s(P) + (j * NP + i) * 2] * gx + T[tab_s(P) + (j * NP + i) * 2 + 1] * gy;
+        for (int i = 0; i < NP; ++i) {
+            r[i] = fma(T[tab_s(P) + (j * NP + i) * 2], gx, r[i]);
+            r[i] = fma(T[tab_s(P) + (j * NP + i) * 2 + 1], gy, r[i]);
+        }
     }
 }
 

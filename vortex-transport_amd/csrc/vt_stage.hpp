@@ -76,9 +76,11 @@ __device__ __forceinline__ double fast_rsq(double x) {
 // C2 sharing G1/ci); all are reassociations of the reference arithmetic at the
 // ulp level.  Returns the checks' outcome as bits: 1 left state non-physical,
 // 2 right state non-physical, 4 c2 <= 0 (flux_code turns them into the
-// reference's error number).
-__device__ __forceinline__ int roe_flux_bits(const double (&UL)[4], const double (&UR)[4], double nx, double ny,
-                                             double (&F)[4]) {
+// reference's error number).  This form returns twice the flux, F2 = 2 F
+// (the halving is exact, so 0.5 * F2 has the bits of F): the stage kernel
+// folds the 1/2 into its lift weights instead of spending four multiplies.
+__device__ __forceinline__ int roe_flux2_bits(const double (&UL)[4], const double (&UR)[4], double nx, double ny,
+                                              double (&F2)[4]) {
     constexpr double gmi = 0.4;
     // States: u = m/rho, p = (gamma-1)(rho E - (m_x u + m_y v)/2), H = (rho E + p)/rho.
     // One refined reciprocal square root of rho_L rho_R gives both
@@ -136,12 +138,21 @@ __device__ __forceinline__ int roe_flux_bits(const double (&UL)[4], const double
     const double a1 = G1 * ci1;
     const double C1 = ci1 * fma(a1, sl, G2 * s2);
     const double C2 = fma(a1, s2, G2 * sl);
-    // F = (F_L.n + F_R.n)/2 - (l2 du + C1 (1, ui, vi, Hi) + C2 (0, nx, ny, ucp))/2
+    // 2 F = (F_L.n + F_R.n) - (l2 du + C1 (1, ui, vi, Hi) + C2 (0, nx, ny, ucp))
     const double pp = pL + pR;
-    F[0] = 0.5 * (fma(rL, unL, rR * unR) - fma(l2, du0, C1));
-    F[1] = 0.5 * (fma(UL[1], unL, fma(UR[1], unR, pp * nx)) - fma(l2, du1, fma(C1, ui, C2 * nx)));
-    F[2] = 0.5 * (fma(UL[2], unL, fma(UR[2], unR, pp * ny)) - fma(l2, du2, fma(C1, vi, C2 * ny)));
-    F[3] = 0.5 * (fma(rHL, unL, rHR * unR) - fma(l2, du3, fma(C1, Hi, C2 * ucp)));
+    F2[0] = fma(rL, unL, rR * unR) - fma(l2, du0, C1);
+    F2[1] = fma(UL[1], unL, fma(UR[1], unR, pp * nx)) - fma(l2, du1, fma(C1, ui, C2 * nx));
+    F2[2] = fma(UL[2], unL, fma(UR[2], unR, pp * ny)) - fma(l2, du2, fma(C1, vi, C2 * ny));
+    F2[3] = fma(rHL, unL, rHR * unR) - fma(l2, du3, fma(C1, Hi, C2 * ucp));
+    return bits;
+}
+
+// The Roe flux F itself (roe_flux2_bits halved).
+__device__ __forceinline__ int roe_flux_bits(const double (&UL)[4], const double (&UR)[4], double nx, double ny,
+                                             double (&F)[4]) {
+    const int bits = roe_flux2_bits(UL, UR, nx, ny, F);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) F[k] *= 0.5;
     return bits;
 }
 
@@ -304,9 +315,14 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         for (int k = 0; k < 4; ++k) {
             const double gx = a00 * fx[k] + a01 * fy[k];
             const double gy = a10 * fx[k] + a11 * fy[k];
+            // Two fused multiply-adds into the accumulator per (i, k), instead
+            // of forming the two products' sum first and adding it (a multiply,
+            // a fused multiply-add and an add).
 #pragma unroll
-            for (int i = 0; i < NP; ++i)
-                r[i][k] += T[tab_s(P) + (j * NP + i) * 2] * gx + T[tab_s(P) + (j * NP + i) * 2 + 1] * gy;
+            for (int i = 0; i < NP; ++i) {
+                r[i][k] = fma(T[tab_s(P) + (j * NP + i) * 2], gx, r[i][k]);
+                r[i][k] = fma(T[tab_s(P) + (j * NP + i) * 2 + 1], gy, r[i][k]);
+            }
         }
     }
 
@@ -329,12 +345,15 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
         const ctab_t tp = T + tab_tp(P) + s * NQ * NE;
         const ctab_t tm = T + tab_tm(P, NQ) + s * NQ * NE;
         const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;
-        auto lift = [&](const double (&tw)[NE], const double (&F)[4]) {
+        // The lift of 2 F with weights len/2 * w: the same products as
+        // len * w * F bit for bit (halving and doubling are exact).
+        const double hlen = 0.5 * len;
+        auto lift = [&](const double (&tw)[NE], const double (&F2)[4]) {
 #pragma unroll
             for (int m = 0; m < NE; ++m) {
-                const double w = len * tw[m];
+                const double w = hlen * tw[m];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) r[edge_node<P>(s, m)][k] -= w * F[k];
+                for (int k = 0; k < 4; ++k) r[edge_node<P>(s, m)][k] -= w * F2[k];
             }
         };
         // On a paired mesh side s = 1 is the diagonal of the element's cell,
@@ -377,17 +396,18 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                 ctl[m] = tl[q * NE + m];
                 crl[m] = tl[(NQ - 1 - q) * NE + m];
             }
-            const int bits = roe_flux_bits(UL, UR, nxv, nyv, F);
+            // The error number is formed only on the (rare) failing lanes:
+            // flux_code is non-zero exactly when its bits are.
+            const int bits = roe_flux2_bits(UL, UR, nxv, nyv, F);
             if (half && q != NQ - 1 - q) {
                 double G[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) G[k] = -lane_pair_swap(F[k]);
-                const int rcode = flux_code(swap_sides(lane_pair_swap(bits)));
-                if (rcode) report(prm.err, prm.stage_id, prm.err_base + e, NQ - 1 - q, s, rcode);
+                const int pbits = lane_pair_swap(bits);
+                if (pbits) report(prm.err, prm.stage_id, prm.err_base + e, NQ - 1 - q, s, flux_code(swap_sides(pbits)));
                 lift(crl, G);
             }
-            const int code = flux_code(bits);
-            if (code) report(prm.err, prm.stage_id, prm.err_base + e, q, s, code);
+            if (bits) report(prm.err, prm.stage_id, prm.err_base + e, q, s, flux_code(bits));
             lift(ctl, F);
         }
         if (s < 2) {
