@@ -279,7 +279,10 @@ def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
     """Forward + discrete adjoint (checkpoints every 32 steps) of the
     mass-weighted L2 misfit to the advected target vortex (eps 0.32, centre
     (0.1, -0.05)) on n x n cells of a 10 x 10 domain, p = 3, against the plain
-    forward solve of the same steps; medians of three runs of each."""
+    forward solve of the same steps; medians of three runs of each.  The
+    reverse sweep recomputes the next segment on a second stream while it
+    sweeps the current one (the library default); the one-stream sweep of the
+    same run is timed beside it."""
     import torch
     mesh = vt.generate_mesh(nx=n, ny=n)
     s = vt.Solver(mesh, 3, 8)
@@ -288,7 +291,7 @@ def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
     U0 = s.state()
     s.rk4(dt, 64)
     # three alternating runs of each, medians reported
-    tfs, tas = [], []
+    tfs, tas, tss = [], [], []
     for _ in range(3):
         s.set_state(U0)
         torch.cuda.synchronize()
@@ -299,6 +302,13 @@ def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
         t0 = time.perf_counter()
         J, lam0 = s.adjoint(dt, steps, tgt, every)
         tas.append(time.perf_counter() - t0)
+        s.set_adjoint_overlap(False)
+        s.set_state(U0)
+        t0 = time.perf_counter()
+        J1, lam1 = s.adjoint(dt, steps, tgt, every)
+        tss.append(time.perf_counter() - t0)
+        s.set_adjoint_overlap(True)
+    serial_equal = bool(J1 == J and np.array_equal(lam1, lam0))
     # The same adjoint with every stage state of the run kept in HBM (one
     # segment of 4 x steps states, 168 GB at 256^2 x 1000 steps): no recompute.
     s.set_state(U0)
@@ -313,6 +323,8 @@ def adjoint_figure(vt, steps, dt=1e-4, n=256, every=32):
                         "(eps 0.32, centre (0.1, -0.05))" % (n, n, steps, dt, every),
             "elements": 2 * n * n, "forward_s": tf, "forward_plus_adjoint_s": ta, "ratio": ta / tf,
             "runs": {"forward_s": tfs, "forward_plus_adjoint_s": tas}, "J": J,
+            "one_stream_sweep": {"forward_plus_adjoint_s": float(np.median(tss)),
+                                 "ratio": float(np.median(tss)) / tf, "runs": tss, "same_bits": serial_equal},
             "grad_norm": float(np.linalg.norm(lam0)),
             "all_stages_stored": {"every": steps, "stage_states_gb": stored_bytes / 1e9,
                                   "forward_plus_adjoint_s": t_stored, "ratio": t_stored / tf,

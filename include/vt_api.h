@@ -216,6 +216,12 @@ int vt_adjoint_recording(const vt_ctx *ctx, unsigned long long *id);
  * checkpoints; 4*every + 1 states of device memory.  The current state is
  * not changed. */
 int vt_adjoint_backward(vt_ctx *ctx, const double *lamT, double *lam0);
+/* on != 0 (the default): the reverse sweep recomputes the next segment's
+ * stage states on a second stream while the adjoint kernels sweep the
+ * current one, in a second set of 4*every states allocated when it fits
+ * beside everything else with 2 GiB to spare (else the sweep runs on one
+ * stream).  Results are the same either way (no reference counterpart). */
+int vt_set_adjoint_overlap(vt_ctx *ctx, int on);
 /* dU0/d(epsilon, x0, y0) of vt_init_vortex type 0 at time t: dU [3][n][Np][4]. */
 int vt_vortex_tangent(const vt_ctx *ctx, double t, double epsilon, double x0, double y0, double *dU);
 /* J = sum_e detJ_e (rho_e - rho_target_e)^T M_ref (rho_e - rho_target_e) of the

@@ -261,3 +261,28 @@ def test_stored_last_segment_equals_recompute(vt, gpu, every):
     assert np.array_equal(a, b)
     assert np.array_equal(a, ref)          # the same kernels on the same states
     s.close()
+
+
+@pytest.mark.parametrize("every", [5, 13])
+def test_overlapped_recompute_equals_serial_sweep(vt, gpu, every):
+    """The reverse sweep recomputing the next segment on a second stream
+    (two alternating slot sets) gives the bits of the one-stream sweep, on a
+    recording with several segments, for a first sweep (last segment stored
+    by the recording) and a repeated one (every segment recomputed)."""
+    dt, nsteps = 1e-3, 40
+    mesh = vt.generate_mesh(nx=12, ny=12, perturb_grid=1.0)
+    s = vt.Solver(mesh, 3, 8)
+    s.initialize_hydrodinamics(0)
+    U0 = s.state()
+    lamT = np.random.default_rng(5).standard_normal(U0.shape)
+    out = {}
+    for on in (False, True):
+        s.set_adjoint_overlap(on)
+        s.set_state(U0)
+        s.time = 0.0
+        s.adjoint_forward(dt, nsteps, every)
+        out[on] = (s.adjoint_backward(lamT), s.adjoint_backward(lamT))
+    assert np.array_equal(out[True][0], out[False][0])
+    assert np.array_equal(out[True][1], out[False][1])
+    assert np.array_equal(out[True][0], out[True][1])
+    s.close()

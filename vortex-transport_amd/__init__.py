@@ -110,6 +110,7 @@ def lib():
             "vt_adjoint_forward": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
             "vt_adjoint_backward": (ctypes.c_int, [vp, _dp, _dp]),
             "vt_adjoint_recording": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_ulonglong)]),
+            "vt_set_adjoint_overlap": (ctypes.c_int, [vp, ctypes.c_int]),
             "vt_vortex_tangent": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
             "vt_mass_l2_objective": (ctypes.c_int, [vp, _dp, _dp, _dp]),
             "vt_adjoint": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
@@ -445,6 +446,11 @@ class Solver:
         r = ctypes.c_ulonglong()
         _check(lib().vt_adjoint_recording(self._h, ctypes.byref(r)))
         return r.value
+
+    def set_adjoint_overlap(self, on):
+        """Whether the reverse sweep recomputes the next segment on a second
+        stream while sweeping the current one (default on; the same results)."""
+        _check(lib().vt_set_adjoint_overlap(self._h, int(bool(on))))
 
     def adjoint_backward(self, lamT):
         """(dU_T/dU_0)^T lamT over the last recorded forward run."""

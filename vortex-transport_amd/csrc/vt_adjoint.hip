@@ -119,7 +119,7 @@ int download_planes(vt_ctx* c, const double* dev, double* U) { return vti::downl
 // step does not read) is skipped.  Failures carry the stage ids
 // stage_base + 0..3.
 hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* S2, double* S3, double* next,
-                          int stage_base = 0) {
+                          int stage_base, hipStream_t st) {
     vt::StageParams prm{};
     prm.vtx = c->vtx;
     prm.nbr = c->nbr;
@@ -139,7 +139,7 @@ hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* 
         prm.Un = (s == 1 || s == 2) ? S0 : nullptr;
         prm.mode = vt::kModeRk1 + s;
         prm.stage_id = stage_base + s;
-        hipError_t e = vt::launch_stage(prm, c->p, c->nq, c->cfg, c->stream);
+        hipError_t e = vt::launch_stage(prm, c->p, c->nq, c->cfg, st);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -158,6 +158,34 @@ int ensure_seg(vt_ctx* c, int nsteps, int every) {
     return VT_OK;
 }
 
+// Headroom left on the device when the second slot set is allocated.
+constexpr size_t kOverlapHeadroom = size_t(2) << 30;
+
+// The second slot set (as large as seg), the recompute stream and its events.
+// Returns false, and the sweep runs on one stream, when the set does not fit
+// beside everything else (the 32-instance ensemble adjoint does not).
+bool ensure_overlap(vt_ctx* c) {
+    if (!c->rstream) {
+        if (!hip_ok(hipStreamCreateWithFlags(&c->rstream, hipStreamNonBlocking), "hipStreamCreate")) return false;
+        for (hipEvent_t* ev : {&c->ev_fork, &c->ev_rec[0], &c->ev_rec[1], &c->ev_swept[0], &c->ev_swept[1]})
+            if (!hip_ok(hipEventCreateWithFlags(ev, hipEventDisableTiming), "hipEventCreate")) return false;
+    }
+    if (c->seg2 && c->seg2_slots == c->seg_slots) return true;
+    if (c->seg2) (void)hipFree(c->seg2);
+    c->seg2 = nullptr;
+    c->seg2_slots = 0;
+    const size_t bytes = sizeof(double) * slot_elems(c) * c->seg_slots;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < bytes + kOverlapHeadroom) return false;
+    if (hipMalloc(reinterpret_cast<void**>(&c->seg2), bytes) != hipSuccess) {
+        c->seg2 = nullptr;
+        (void)hipGetLastError();
+        return false;
+    }
+    c->seg2_slots = c->seg_slots;
+    return true;
+}
+
 }  // namespace
 
 int vti::backward_sweep(vt_ctx* c) {
@@ -166,6 +194,7 @@ int vti::backward_sweep(vt_ctx* c) {
     const size_t n = slot_elems(c);
     int rc = ensure_seg(c, nsteps, every);
     if (rc) return rc;
+    const bool overlap = c->adj_overlap && nseg > 1 && ensure_overlap(c);
     double *lam = c->adj, *acc = c->adj + n, *y[2] = {c->adj + 2 * n, c->adj + 3 * n};
     vt::AdjParams a = adj_params(c, vt::kAdjSeed);
     a.lam = lam;
@@ -173,34 +202,75 @@ int vti::backward_sweep(vt_ctx* c) {
     a.scale = 1.0 / 6.0;
     HIP_TRY(vt::launch_adjoint(a, c->p, c->nq, c->stream));
     int cur = 0;
-    // The segment's first step starts from its checkpoint in place.
-    int seg_i = 0;
-    auto slot = [&](int step, int stage) {
-        return (step == 0 && stage == 0) ? c->ckpt[seg_i] : c->seg + (static_cast<size_t>(4) * step + stage) * n;
+    // Segment i's stage states: its first step starts from its checkpoint in
+    // place; the rest sit in seg, or with overlap in seg and seg2 alternately
+    // (the last segment, which the recording may have stored, in seg).
+    auto set_of = [&](int i) { return (overlap && ((nseg - 1 - i) & 1)) ? c->seg2 : c->seg; };
+    auto slot = [&](int i, int step, int stage) {
+        return (step == 0 && stage == 0) ? c->ckpt[i] : set_of(i) + (static_cast<size_t>(4) * step + stage) * n;
     };
-    for (int i = nseg - 1; i >= 0; --i) {
-        const int len = (i + 1 < nseg) ? every : nsteps - i * every;
-        seg_i = i;
-        // The last segment's stage states were kept by the recording itself
-        // (until a recompute of an earlier segment overwrites them).
-        if (!(i == nseg - 1 && c->rec_last_stored)) {
-            c->rec_last_stored = false;
-            for (int s = 0; s < len; ++s)
-                HIP_TRY(recompute_step(c, dt, slot(s, 0), slot(s, 1), slot(s, 2), slot(s, 3),
-                                       s + 1 < len ? slot(s + 1, 0) : nullptr));
+    auto seg_len = [&](int i) { return (i + 1 < nseg) ? every : nsteps - i * every; };
+    auto recompute = [&](int i, hipStream_t st) -> hipError_t {
+        const int len = seg_len(i);
+        for (int s = 0; s < len; ++s) {
+            const hipError_t e = recompute_step(c, dt, slot(i, s, 0), slot(i, s, 1), slot(i, s, 2), slot(i, s, 3),
+                                                s + 1 < len ? slot(i, s + 1, 0) : nullptr, 0, st);
+            if (e != hipSuccess) return e;
         }
-        for (int s = len - 1; s >= 0; --s)
+        return hipSuccess;
+    };
+    auto sweep = [&](int i) -> hipError_t {
+        for (int s = seg_len(i) - 1; s >= 0; --s)
             for (int st = 3; st >= 0; --st) {
                 vt::AdjParams b = adj_params(c, vt::kAdj1 + st);
-                b.U = slot(s, st);
+                b.U = slot(i, s, st);
                 b.Y = y[cur];
                 b.Yout = y[cur ^ 1];
                 b.lam = lam;
                 b.acc = acc;
                 b.dt = dt;
-                HIP_TRY(vt::launch_adjoint(b, c->p, c->nq, c->stream));
+                const hipError_t e = vt::launch_adjoint(b, c->p, c->nq, c->stream);
+                if (e != hipSuccess) return e;
                 cur ^= 1;
             }
+        return hipSuccess;
+    };
+    // The last segment's stage states were kept by the recording itself
+    // (until a recompute of an earlier segment overwrites them).
+    const bool stored = c->rec_last_stored;
+    if (!overlap) {
+        for (int i = nseg - 1; i >= 0; --i) {
+            if (!(i == nseg - 1 && stored)) {
+                c->rec_last_stored = false;
+                HIP_TRY(recompute(i, c->stream));
+            }
+            HIP_TRY(sweep(i));
+        }
+        return VT_OK;
+    }
+    // Overlapped: while segment i is swept on the context stream, segment
+    // i - 1 is recomputed on rstream into the other set, once the sweep of
+    // segment i + 1 (the set's previous user) is done; the sweep of segment
+    // i - 1 waits for its recompute.  The recompute stream starts after all
+    // work queued so far on the context stream.
+    c->rec_last_stored = false;
+    HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->rstream, c->ev_fork, 0));
+    if (!stored) {
+        HIP_TRY(recompute(nseg - 1, c->rstream));
+        HIP_TRY(hipEventRecord(c->ev_rec[0], c->rstream));
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_rec[0], 0));
+    }
+    for (int i = nseg - 1; i >= 0; --i) {
+        const int b = (nseg - 1 - i) & 1;
+        if (i > 0) {
+            if (i + 1 < nseg) HIP_TRY(hipStreamWaitEvent(c->rstream, c->ev_swept[b ^ 1], 0));
+            HIP_TRY(recompute(i - 1, c->rstream));
+            HIP_TRY(hipEventRecord(c->ev_rec[b ^ 1], c->rstream));
+        }
+        HIP_TRY(sweep(i));
+        HIP_TRY(hipEventRecord(c->ev_swept[b], c->stream));
+        if (i > 0) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_rec[b ^ 1], 0));
     }
     return VT_OK;
 }
@@ -210,9 +280,18 @@ void vti::free_adjoint(vt_ctx* c) {
         if (p) (void)hipFree(p);
     c->ckpt.clear();
     if (c->seg) (void)hipFree(c->seg);
+    if (c->seg2) (void)hipFree(c->seg2);
     if (c->adj) (void)hipFree(c->adj);
     for (double* p : {c->mref, c->tgt, c->part})
         if (p) (void)hipFree(p);
+    for (hipEvent_t* ev : {&c->ev_fork, &c->ev_rec[0], &c->ev_rec[1], &c->ev_swept[0], &c->ev_swept[1]}) {
+        if (*ev) (void)hipEventDestroy(*ev);
+        *ev = nullptr;
+    }
+    if (c->rstream) (void)hipStreamDestroy(c->rstream);
+    c->rstream = nullptr;
+    c->seg2 = nullptr;
+    c->seg2_slots = 0;
     c->seg = nullptr;
     c->adj = nullptr;
     c->mref = c->tgt = c->part = nullptr;
@@ -292,7 +371,7 @@ int vt_adjoint_forward(vt_ctx* c, double dt, int nsteps, int every) {
         HIP_TRY(hipMemcpyAsync(c->err, c->err_init, sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
         for (int s = 0; s < len; ++s)
             HIP_TRY(recompute_step(c, dt, slot(s, 0), slot(s, 1), slot(s, 2), slot(s, 3),
-                                   s + 1 < len ? slot(s + 1, 0) : c->Un, 4 * s));
+                                   s + 1 < len ? slot(s + 1, 0) : c->Un, 4 * s, c->stream));
         HIP_TRY(hipMemcpyAsync(c->err_host, c->err, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (c->err_host[0] != vt::kNoError) {
@@ -306,6 +385,12 @@ int vt_adjoint_forward(vt_ctx* c, double dt, int nsteps, int every) {
     c->rec_every = every;
     c->rec_dt = dt;
     c->recorded = true;
+    return VT_OK;
+}
+
+int vt_set_adjoint_overlap(vt_ctx* c, int on) {
+    if (!c) return VT_ERR_ARG;
+    c->adj_overlap = on != 0;
     return VT_OK;
 }
 

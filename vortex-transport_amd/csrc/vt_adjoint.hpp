@@ -32,8 +32,10 @@ namespace vt {
 
 // (dF/dUL)^T bF for the Roe flux of Numericalflux.cpp:31-156, differentiated on
 // the branches the forward takes (entropy fix, |.|); reciprocals as in roe_flux.
+// The cotangent comes halved, hF = bF / 2: every path from F to the state
+// carries the flux's factor 1/2, which the caller folds into its weights.
 __device__ __forceinline__ void roe_flux_vjp_left(const double (&UL)[4], const double (&UR)[4], double nx,
-                                                  double ny, const double (&bF)[4], double (&bUL)[4]) {
+                                                  double ny, const double (&hF)[4], double (&bUL)[4]) {
     constexpr double gmi = 0.4;
     // forward sweep
     const double rL = UL[0], irL = fast_rcp(rL);
@@ -64,11 +66,11 @@ __device__ __forceinline__ void roe_flux_vjp_left(const double (&UL)[4], const d
     const double C1 = G1 * sl * ci1 * ci1 + G2 * s2 * ci1;
     const double C2 = G1 * s2 * ci1 + G2 * sl;
     // reverse sweep
-    const double bl3a = -0.5 * (bF[0] * du0 + bF[1] * du1 + bF[2] * du2 + bF[3] * du3);
-    double bdu0 = -0.5 * l3 * bF[0], bdu1 = -0.5 * l3 * bF[1], bdu2 = -0.5 * l3 * bF[2], bdu3 = -0.5 * l3 * bF[3];
-    const double bC1 = -0.5 * (bF[0] + bF[1] * ui + bF[2] * vi + bF[3] * Hi);
-    const double bC2 = -0.5 * (bF[1] * nx + bF[2] * ny + bF[3] * ucp);
-    double bui = -0.5 * C1 * bF[1], bvi = -0.5 * C1 * bF[2], bHi = -0.5 * C1 * bF[3], bucp = -0.5 * C2 * bF[3];
+    const double bl3a = -(hF[0] * du0 + hF[1] * du1 + hF[2] * du2 + hF[3] * du3);
+    double bdu0 = -l3 * hF[0], bdu1 = -l3 * hF[1], bdu2 = -l3 * hF[2], bdu3 = -l3 * hF[3];
+    const double bC1 = -(hF[0] + hF[1] * ui + hF[2] * vi + hF[3] * Hi);
+    const double bC2 = -(hF[1] * nx + hF[2] * ny + hF[3] * ucp);
+    double bui = -C1 * hF[1], bvi = -C1 * hF[2], bHi = -C1 * hF[3], bucp = -C2 * hF[3];
     // C2 = G1 s2 ci1 + G2 (s1 - l3);  C1 = G1 (s1 - l3) ci1^2 + G2 s2 ci1
     const double bG1 = bC2 * s2 * ci1 + bC1 * sl * ci1 * ci1;
     const double bG2 = bC2 * sl + bC1 * s2 * ci1;
@@ -111,7 +113,7 @@ __device__ __forceinline__ void roe_flux_vjp_left(const double (&UL)[4], const d
     const double bdi = (bui * uR + bvi * vR + bHi * HR) * d1 - bd1 * d1 * d1;
     double brL = -0.5 * bdi * di * irL;
     // left-state flux F_L = (rL unL, UL1 unL + pL nx, UL2 unL + pL ny, rHL unL), weight 1/2
-    const double h0 = 0.5 * bF[0], h1 = 0.5 * bF[1], h2 = 0.5 * bF[2], h3 = 0.5 * bF[3];
+    const double h0 = hF[0], h1 = hF[1], h2 = hF[2], h3 = hF[3];
     brL += h0 * unL;
     const double bun = h0 * rL + h1 * UL[1] + h2 * UL[2] + h3 * rHL;
     double bU1 = h1 * unL, bU2 = h2 * unL;
@@ -262,7 +264,7 @@ __global__ void __launch_bounds__(kAdjBlock, 1) adjoint_stage_kernel(AdjParams p
         const double dx = vx[(s + 1) % 3] - vx[s], dy = vy[(s + 1) % 3] - vy[s];
         // |side| and the unit normal from one refined 1/|side|.
         const double l2s = dx * dx + dy * dy, il = fast_rsq(l2s);
-        const double len = l2s * il, nxv = dy * il, nyv = -dx * il;
+        const double hlen = 0.5 * (l2s * il), nxv = dy * il, nyv = -dx * il;
         const ctab_t tp = T + tab_tp(P) + s * NQ * NE;
         const ctab_t tm = T + tab_tm(P, NQ) + s * NQ * NE;
         const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;
@@ -279,8 +281,8 @@ __global__ void __launch_bounds__(kAdjBlock, 1) adjoint_stage_kernel(AdjParams p
             }
 #pragma unroll 1
         for (int q = 0; q < NQ; ++q) {
-            // d(q) = len (c_e(q) - c_nb(q~)), the cotangent of this element's
-            // flux minus that of the neighbour's flux at the same point.
+            // d(q) = len (c_e(q) - c_nb(q~)) / 2, half the cotangent of this
+            // element's flux minus that of the neighbour's flux at the same point.
             const int qn = NQ - 1 - q;
             double UL[4], UR[4], d[4], bUL[4];
 #pragma unroll
@@ -295,7 +297,7 @@ __global__ void __launch_bounds__(kAdjBlock, 1) adjoint_stage_kernel(AdjParams p
                 }
                 UL[k] = a;
                 UR[k] = c;
-                d[k] = len * (ce - cn);
+                d[k] = hlen * (ce - cn);
             }
             roe_flux_vjp_left(UL, UR, nxv, nyv, d, bUL);
 #pragma unroll

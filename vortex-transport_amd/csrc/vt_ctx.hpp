@@ -46,6 +46,14 @@ struct vt_ctx {
     bool recorded = false;
     bool rec_last_stored = false;            // seg holds the last segment's stage states from the recording
     unsigned long long rec_id = 0;           // number of vt_adjoint_forward calls so far
+    // A second segment slot set and a stream of its own: when both sets fit in
+    // device memory, the reverse sweep recomputes the next segment's stage
+    // states on rstream while the adjoint kernels sweep the current segment.
+    bool adj_overlap = true;
+    double* seg2 = nullptr;
+    int seg2_slots = 0;
+    hipStream_t rstream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_swept[2] = {nullptr, nullptr};
     // Device set-up of the vortex state: area rule, basis values and Minv_ref
     // of the least-squares type (uploaded on first use), and the objective's
     // target vortex {t, eps, x0, y0} set by vt_set_target_vortex.
