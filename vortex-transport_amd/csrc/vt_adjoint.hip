@@ -166,9 +166,19 @@ constexpr size_t kOverlapHeadroom = size_t(2) << 30;
 // beside everything else (the 32-instance ensemble adjoint does not).
 bool ensure_overlap(vt_ctx* c) {
     if (!c->rstream) {
-        if (!hip_ok(hipStreamCreateWithFlags(&c->rstream, hipStreamNonBlocking), "hipStreamCreate")) return false;
-        for (hipEvent_t* ev : {&c->ev_fork, &c->ev_rec[0], &c->ev_rec[1], &c->ev_swept[0], &c->ev_swept[1]})
-            if (!hip_ok(hipEventCreateWithFlags(ev, hipEventDisableTiming), "hipEventCreate")) return false;
+        hipEvent_t* evs[] = {&c->ev_fork, &c->ev_rec[0], &c->ev_rec[1], &c->ev_swept[0], &c->ev_swept[1]};
+        bool ok = hip_ok(hipStreamCreateWithFlags(&c->rstream, hipStreamNonBlocking), "hipStreamCreate");
+        for (hipEvent_t* ev : evs)
+            ok = ok && hip_ok(hipEventCreateWithFlags(ev, hipEventDisableTiming), "hipEventCreate");
+        if (!ok) {  // all or nothing, so that a later call starts over
+            for (hipEvent_t* ev : evs) {
+                if (*ev) (void)hipEventDestroy(*ev);
+                *ev = nullptr;
+            }
+            if (c->rstream) (void)hipStreamDestroy(c->rstream);
+            c->rstream = nullptr;
+            return false;
+        }
     }
     if (c->seg2 && c->seg2_slots == c->seg_slots) return true;
     if (c->seg2) (void)hipFree(c->seg2);
