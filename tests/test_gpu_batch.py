@@ -145,8 +145,10 @@ def test_batch_set_state_roundtrip(vt, gpu):
         assert np.array_equal(b.state(i), X[i])
 
 
-def test_batch_nonphysical_names_the_instance(vt, oracle, gpu):
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_batch_nonphysical_names_the_instance(vt, oracle, gpu, concurrent):
     meshes, b = _build(vt, SMALL)
+    b.set_launch(64, concurrent=concurrent)
     k = 4
     U = b.state(k)
     U[11, 3, 0] = -1.0
