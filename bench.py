@@ -66,6 +66,8 @@ def parse_args():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--orders", action="store_true",
                     help="also time the 512 x 512 mesh at p = 1 and 2 (ns per element-stage per order)")
+    ap.add_argument("--size-sweep", action="store_true",
+                    help="per-order ms per step of single meshes from 32^2 to 1024^2 cells and the ensemble's speed-up ceiling")
     ap.add_argument("--batched-by-order", action="store_true",
                     help="also time the ensemble's instances of each order p batched alone, against their plain runs")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -408,6 +410,60 @@ def orders_figure(vt, mesh, dt, steps=256):
     return out
 
 
+SWEEP_CELLS = (32, 64, 96, 128, 181, 256, 362, 512, 724, 1024)
+
+
+def size_sweep_figure(vt, dt=2e-5):
+    """Cost of the per-stage kernels against mesh size, per order: ms per RK4
+    step and ns per element-stage of single n x n-cell meshes (10 x 10 domain,
+    nodal vortex) for p = 1, 2, 3, medians of three timed runs of 16·k steps
+    (graphs built and 0.2 s of untimed steps first).
+
+    From these rates, the ceiling of the ensemble's speed-up: the 64 instances
+    of the batched figure (seed 3) stepped one plain solver after another cost
+    sum_i 4 ne_i r_p(ne_i) per step (rate interpolated in log ne between the
+    measured sizes), and the same elements each at the rate of its order on the
+    largest mesh cost sum_i 4 ne_i r_p(2 * 1024^2); the quotient is the speed-up
+    a batched step running every element at large-mesh efficiency shows."""
+    import torch
+    rates = {}
+    out = {}
+    for p in (1, 2, 3):
+        pts = []
+        for n in SWEEP_CELLS:
+            mesh = vt.generate_mesh(nx=n, ny=n)
+            s = vt.Solver(mesh, p, 8)
+            s.initialize_hydrodinamics(0, device=True)
+            steps = 16 * max(2, min(64, (1 << 20) // s.ne))
+            s.prepare(1, dt, steps)
+            settle_clocks(s, dt, 0.2)
+            ws = []
+            for _ in range(3):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                s.rk4(dt, steps)
+                torch.cuda.synchronize()
+                ws.append(time.perf_counter() - t0)
+            w = float(np.median(ws))
+            pts.append((s.ne, w / steps / 4 / s.ne * 1e9))
+            out["p%d_%d" % (p, n)] = {"elements": s.ne, "steps": steps, "ms_per_step": w / steps * 1e3,
+                                      "ns_per_element_stage": w / steps / 4 / s.ne * 1e9}
+            s.close()
+        rates[p] = pts
+    draws = vt.ensemble_draws(3, 64)
+
+    def rate(p, ne):
+        x = np.log([q[0] for q in rates[p]])
+        return float(np.interp(np.log(ne), x, [q[1] for q in rates[p]]))
+
+    per_inst = sum(4 * 2 * d["cells"] ** 2 * rate(d["p"], 2 * d["cells"] ** 2) for d in draws) * 1e-6
+    at_large = sum(4 * 2 * d["cells"] ** 2 * rates[d["p"]][-1][1] for d in draws) * 1e-6
+    out["ensemble_seed3_64"] = {"per_instance_ms_per_step_from_rates": per_inst,
+                                "all_at_largest_mesh_rate_ms_per_step": at_large,
+                                "speedup_ceiling": per_inst / at_large}
+    return out
+
+
 def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
     """Forward-only p = 3 run on n x n cells (2 n^2 triangles) of the 20 x 20
     perturbed mesh (seed 2): device-resident state sizing and throughput."""
@@ -677,6 +733,8 @@ def main():
         line["orders"] = orders_figure(vt, mesh, a.dt)
     if a.batched_by_order:
         line["batched_by_order"] = batched_by_order_figure(vt, a.batched_steps)
+    if a.size_sweep:
+        line["size_sweep"] = size_sweep_figure(vt)
     if not a.no_adjoint:
         line["adjoint"] = adjoint_figure(vt, a.adjoint_steps)
     if not a.no_batched_adjoint:
