@@ -17,7 +17,9 @@ ensemble, one launch per RK stage, against per-instance plain solvers),
 the forward solve alone), "batched_adjoint" (32 uneven instances up to
 512 x 512 cells, recorded forward + reverse sweep + per-instance parameter
 gradients, against the batched forward) and "large_mesh" (a forward-only
-1024 x 1024-cell run).
+1024 x 1024-cell run), "c1_full_run" and "shipped_input_run" (the
+reference's own 8 x 8 case to the end at integration orders 8 and 6, the
+latter its shipped input, beside the rk4 times the reference recorded for it).
 
 Prints one JSON line (rank 0).  --gpus N > 1 exits: the implementation runs on one GPU.
 """
@@ -549,17 +551,26 @@ def shard_figure(vt, steps, dt=1e-4, nx=512, ny=64):
             "value": ne * 10 * 4 * 4 * steps / wall, "unit": "DOF-stage updates/s"}
 
 
-def c1_figure(vt, with_cpu):
+# The reference's own recorded serial profiles of its shipped input
+# (input_files/input: 8 x 8 cells, p = 3, integration_order 6, 10,000 RK4
+# steps), rk4 inclusive seconds (performance_history/profile_88dbe79_*.txt:90,
+# profile_4d2e0da_*.txt:90, profile_16a08ee_*.txt:10098; CPU not recorded).
+REFERENCE_RECORDED_RK4_S = {"88dbe79": 32.349, "4d2e0da": 29.354, "16a08ee": 35.736}
+
+
+def c1_figure(vt, with_cpu, order=8):
     """The reference's own small case run to the end: 8 x 8 cells of the
-    10 x 10 domain (128 triangles), p = 3, integration_order 8, 10,000 RK4
-    steps of dt = 1.414213562373e-3 (T = 10 sqrt 2), time to solution on the
-    GPU and, beside it, the reference's translation units on the host."""
+    10 x 10 domain (128 triangles), p = 3, 10,000 RK4 steps of
+    dt = 1.414213562373e-3 (T = 10 sqrt 2) at the given integration order
+    (8: the C1 configuration; 6: the reference's shipped input), time to
+    solution on the GPU and, beside it, the reference's translation units on
+    the host."""
     import torch
-    prm = {"p": 3, "nx": 8, "ny": 8, "integration_order": 8, "simulation_time": "14.14213562373",
+    prm = {"p": 3, "nx": 8, "ny": 8, "integration_order": order, "simulation_time": "14.14213562373",
            "number_time_steps": 10000}
     dt = float(prm["simulation_time"]) / prm["number_time_steps"]
     mesh = vt.generate_mesh(nx=8, ny=8)
-    s = vt.Solver(mesh, 3, 8)
+    s = vt.Solver(mesh, 3, order)
     s.initialize_hydrodinamics(0)
     s.rk4(dt, 64)
     s.time = 0.0  # the initial state is set at the solver's time
@@ -571,8 +582,11 @@ def c1_figure(vt, with_cpu):
     U0 = s.exact_vortex(0.0)
     err = s.l2_state_error(U0)
     s.close()
-    out = {"workload": "8x8 cells (128 triangles), 10x10 domain, p=3, integration_order 8, 10000 RK4 steps of dt %.12g"
-                       % dt, "gpu_s": gpu_s, "l2_state_error": [float(x) for x in err]}
+    out = {"workload": "8x8 cells (128 triangles), 10x10 domain, p=3, integration_order %d, 10000 RK4 steps of dt %.12g"
+                       % (order, dt), "gpu_s": gpu_s, "l2_state_error": [float(x) for x in err]}
+    if order == 6:
+        out["reference_recorded_rk4_s"] = REFERENCE_RECORDED_RK4_S
+        out["speedup_vs_recorded"] = min(REFERENCE_RECORDED_RK4_S.values()) / gpu_s
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -747,6 +761,7 @@ def main():
         line["pcie_inclusive"] = pcie_figure(vt, 5)
     if not a.no_c1:
         line["c1_full_run"] = c1_figure(vt, not a.no_cpu)
+        line["shipped_input_run"] = c1_figure(vt, not a.no_cpu, order=6)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mesh, p, order, a.cpu_steps)
     print(json.dumps(line))
