@@ -278,7 +278,11 @@ __global__ void __launch_bounds__(kSmallThreads, 1) small_run_kernel(SmallParams
                 }
             }
         }
-        __syncthreads();  // every read of S in this stage is done
+        // No barrier before the write: every read of another element's state
+        // (phase A's traces, the volume term) ended at the barrier above, and
+        // phase B reads only this thread's own entries of S.  The barrier after
+        // it orders the write before the next stage's reads, and this stage's
+        // reads of Fs before the next stage's phase A overwrites it.
         if (bact) {
 #pragma unroll
             for (int i = 0; i < NP; ++i) S[(kb * NP + i) * ns + eb] = unew[i];
