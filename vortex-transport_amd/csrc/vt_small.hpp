@@ -245,9 +245,11 @@ __global__ void __launch_bounds__(kSmallThreads, 1) small_run_kernel(SmallParams
                 const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
-                    const double F = Fs[((s * NQ + q) * 4 + kb) * ns + eb];
+                    // len * F once per point, then one FMA per edge node (the
+                    // reference's len * w * phi * F regrouped at the ulp level).
+                    const double lF = len[s] * Fs[((s * NQ + q) * 4 + kb) * ns + eb];
 #pragma unroll
-                    for (int m = 0; m < NE; ++m) r[edge_node<P>(s, m)] -= (len[s] * tl[q * NE + m]) * F;
+                    for (int m = 0; m < NE; ++m) r[edge_node<P>(s, m)] -= tl[q * NE + m] * lF;
                 }
             }
             const double dt = prm.dt;
