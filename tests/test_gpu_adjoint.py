@@ -73,6 +73,25 @@ def test_rk4_adjoint_matches_oracle(vt, oracle, gpu, nsteps, every):
     assert rel(lam0, lo) < 1e-11, rel(lam0, lo)
 
 
+@pytest.mark.parametrize("p,order,n,perturb", [(0, 2, 5, 0.0), (1, 4, 6, 1.0), (1, 19, 5, 0.0), (2, 6, 5, 1.0),
+                                               (2, 13, 6, 0.0), (3, 10, 4, 1.0), (3, 19, 5, 0.0)])
+def test_rk4_adjoint_every_order_matches_oracle(vt, oracle, gpu, p, order, n, perturb):
+    """The reverse sweep at every polynomial order and at 2 to 10 Gauss points
+    per side (integration orders 2 to 19): the ensemble adjoint runs each
+    (p, order) group through these stage kernels."""
+    dt, nsteps, every = 1e-3, 12, 5
+    mesh, s, o = _pair(vt, oracle, n, p, order, perturb)
+    s.initialize_hydrodinamics(0)
+    U0 = s.state()
+    o.set_state(U0)
+    lamT = np.random.default_rng(100 * p + order).standard_normal(U0.shape)
+    s.adjoint_forward(dt, nsteps, every)
+    lam0 = s.adjoint_backward(lamT)
+    lo = o.rk4_adjoint(dt, nsteps, lamT)
+    assert rel(s.state(), o.state()) < 1e-12
+    assert rel(lam0, lo) < 1e-11, rel(lam0, lo)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient(vt, gpu):
     mesh = vt.generate_mesh(nx=12, ny=12)
     s = vt.Solver(mesh, 3, 8)
