@@ -94,6 +94,60 @@ __device__ __forceinline__ void small_side_flux(const double* S, double* Fs, con
     }
 }
 
+// Phase A for two items at once (points of this thread's two rounds), in
+// one instruction stream so that their two flux evaluations interleave.  The
+// side index of each is wavefront-uniform; an item with ok false (past the
+// end of its side group) is evaluated on a valid side and stores nothing.
+template <int P, int NQ, int ns>
+__device__ __forceinline__ void small_side_flux2(const double* S, double* Fs, const double* Nf, const double* Tw,
+                                                 const int* Fc, unsigned long long* key, int nfp, const int (&sd)[2],
+                                                 const int (&f)[2], const int (&q)[2], const bool (&ok)[2], int st) {
+    constexpr int NP = (P + 1) * (P + 2) / 2;
+    constexpr int NE = P + 1;
+    double UL[2][4], UR[2][4], F[2][4];
+    int e[2], nb[2], bits[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int fc = Fc[f[t]];
+        e[t] = fc & 1023;
+        nb[t] = (fc >> 10) & 1023;
+        const double* tp = Tw + (sd[t] * NQ + q[t]) * NE;
+        const double* tm = tp + 3 * NQ * NE;
+        int en[NE];
+#pragma unroll
+        for (int m = 0; m < NE; ++m) en[m] = sd[t] == 0 ? edge_node<P>(0, m) : sd[t] == 1 ? edge_node<P>(1, m) : edge_node<P>(2, m);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            double a = 0.0, b = 0.0;
+#pragma unroll
+            for (int m = 0; m < NE; ++m) {
+                a += tp[m] * S[(k * NP + en[m]) * ns + e[t]];
+                b += tm[m] * S[(k * NP + en[m]) * ns + nb[t]];
+            }
+            UL[t][k] = a;
+            UR[t][k] = b;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) bits[t] = roe_flux_bits(UL[t], UR[t], Nf[f[t]], Nf[nfp + f[t]], F[t]);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        if (!ok[t]) continue;
+        const int qm = NQ - 1 - q[t];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            Fs[((sd[t] * NQ + q[t]) * 4 + k) * ns + e[t]] = F[t][k];
+            Fs[((sd[t] * NQ + qm) * 4 + k) * ns + nb[t]] = -F[t][k];
+        }
+        if (bits[t]) {
+            const unsigned long long stage = static_cast<unsigned long long>(st) << 40;
+            const int ce = flux_code(bits[t]), cn = flux_code(swap_sides(bits[t]));
+            if (ce) atomicMin(key, stage | ((((static_cast<unsigned long long>(e[t]) * 16 + q[t]) * 4 + sd[t]) * 4) + ce));
+            if (cn) atomicMin(key, stage | ((((static_cast<unsigned long long>(nb[t]) * 16 + qm) * 4 + sd[t]) * 4) + cn));
+        }
+    }
+}
+
 // Volume term of variable K at element e: r_i += Sxi_ij G^xi_j + Seta_ij G^eta_j
 // with G = adj(J) F_K(U_j), the nodal flux as the stage kernel forms it.
 template <int P, int K, int ns>
@@ -211,7 +265,26 @@ __global__ void __launch_bounds__(kSmallThreads, 1) small_run_kernel(SmallParams
         // Phase A: one flux evaluation per listed side and Gauss point.  The
         // sides are listed by side index s, each group's items padded to whole
         // wavefronts, so s is uniform over a wavefront.
-        for (int it0 = tid - (tid & 63); it0 < ib[3]; it0 += kSmallThreads) {
+        // Two rounds at a time: while the wavefront's item of the next round
+        // exists, both are evaluated together (small_side_flux2).
+        int it0 = tid - (tid & 63);
+        for (; it0 + kSmallThreads < ib[3]; it0 += 2 * kSmallThreads) {
+            int sd[2], f[2], q[2];
+            bool ok[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int w0 = it0 + t * kSmallThreads;
+                sd[t] = __builtin_amdgcn_readfirstlane((w0 >= ib[1]) + (w0 >= ib[2]));
+                const int li = w0 + (tid & 63) - ib[sd[t]];
+                const int cnt = (prm.fbeg[sd[t] + 1] - prm.fbeg[sd[t]]) * NQ;
+                ok[t] = li < cnt;
+                const int lc = ok[t] ? li : 0;
+                f[t] = prm.fbeg[sd[t]] + lc / NQ;
+                q[t] = lc - (lc / NQ) * NQ;
+            }
+            small_side_flux2<P, NQ, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, sd, f, q, ok, st);
+        }
+        for (; it0 < ib[3]; it0 += kSmallThreads) {
             const int it = it0 + (tid & 63);
             const int sd = __builtin_amdgcn_readfirstlane((it0 >= ib[1]) + (it0 >= ib[2]));
             const int li = it - ib[sd];
