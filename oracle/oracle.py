@@ -13,6 +13,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_RUN = os.path.join(HERE, "_ref", "vt_ref_run")
+# The same translation units at -O3 with AVX2/FMA: a CPU timing baseline only
+# (FMA contraction rounds differently from the reference's profiling build).
+REF_RUN_O3 = os.path.join(HERE, "_ref", "vt_ref_run_o3")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -312,10 +315,12 @@ def write_input(path, p=3, nx=8, ny=8, domain_x="10.0", domain_y="10.0", perturb
 
 
 def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, threads=None, no_state=False,
-                  env_extra=None, write_dumps=False):
-    """Runs oracle/_ref/vt_ref_run (the reference's own TUs) in workdir; returns a dict of arrays."""
-    if not os.path.exists(REF_RUN):
-        raise FileNotFoundError(REF_RUN)
+                  env_extra=None, write_dumps=False, binary=None):
+    """Runs oracle/_ref/vt_ref_run (the reference's own TUs; binary: another
+    build of them, such as REF_RUN_O3) in workdir; returns a dict of arrays."""
+    exe = binary or REF_RUN
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
     p = input_kwargs.get("p", 3)
     order = input_kwargs.get("integration_order", 8)
     npn = np_of(p)
@@ -327,7 +332,7 @@ def run_reference(workdir, input_kwargs, steps, rhs_at_end=False, mesh=None, thr
     inverse_mass_matrix_ref(p, order).tofile(minv)
     stiffness_matrix_ref(p, order).tofile(stiff)
     out = os.path.join(workdir, "out.bin")
-    cmd = [REF_RUN, "run", inp, out, "--minv", minv, "--stiff", stiff, "--steps", str(steps)]
+    cmd = [exe, "run", inp, out, "--minv", minv, "--stiff", stiff, "--steps", str(steps)]
     if rhs_at_end:
         cmd.append("--rhs-at-end")
     if no_state:
