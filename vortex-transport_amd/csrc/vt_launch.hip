@@ -17,13 +17,45 @@ __global__ void flux_kernel(const double* __restrict__ in, double* __restrict__ 
     for (int k = 0; k < 4; ++k) out[4 * static_cast<size_t>(i) + k] = F[k];
 }
 
+// Sum over the 64 lanes of a wavefront by a butterfly of lane swaps: at each
+// level both lanes of a pair add the same two values, so every lane ends with
+// the same bits, in an order fixed by the lane numbers.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// The block's sums of NV values per thread, into out[0..NV): each wavefront's
+// sums by wave_sum, then the BS / 64 wavefront sums added in wavefront order
+// (a fixed shape, so the result is the same run to run).  red holds NV * BS / 64
+// doubles; every thread of the block calls this.
+template <int NV, int BS>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double* red, double* out) {
+    static_assert(BS % 64 == 0 && NV <= BS, "whole wavefronts, one output per thread");
+    constexpr int NW = BS / 64;
+    const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[k * NW + wave] = v[k];
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < NV) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) t += red[threadIdx.x * NW + w];
+        out[threadIdx.x] = t;
+    }
+}
+
 // The objective (see ObjParams).  Per element the sum runs over nodes in order,
-// as (detJ drho_i) (M_ref drho)_i; the block sum is a fixed-shape tree, so J
-// is reproducible run to run.
+// as (detJ drho_i) (M_ref drho)_i; the block sum is block_sums', so J is
+// reproducible run to run.
 template <int P>
 __global__ void __launch_bounds__(kObjBlock) objective_kernel(ObjParams prm) {
     constexpr int NP = (P + 1) * (P + 2) / 2;
-    __shared__ double red[kObjBlock];
+    __shared__ double red[kObjBlock / 64];
     const int e = blockIdx.x * kObjBlock + static_cast<int>(threadIdx.x);
     double je = 0.0;
     if (e < prm.ne) {
@@ -48,23 +80,17 @@ __global__ void __launch_bounds__(kObjBlock) objective_kernel(ObjParams prm) {
 #pragma unroll
         for (int v = NP; v < 4 * NP; ++v) plane_at(prm.lam, v, ld8, oe) = 0.0;
     }
-    red[threadIdx.x] = je;
-    __syncthreads();
-#pragma unroll
-    for (int w = kObjBlock / 2; w > 0; w >>= 1) {
-        if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) prm.part[blockIdx.x] = red[0];
+    double v[1] = {je};
+    block_sums<1, kObjBlock>(v, red, prm.part + blockIdx.x);
 }
 
 // The L2 state-error metric (see L2Params).  Per element and component:
 // sum over the rule's points of w_i * sum_j phi_j(x_i) (U - Uref)_j^2, in that
-// order, times detJ; the block sum is a fixed-shape tree.
+// order, times detJ; the block sums are block_sums'.
 template <int P>
 __global__ void __launch_bounds__(kL2Block) l2_kernel(L2Params prm) {
     constexpr int NP = (P + 1) * (P + 2) / 2;
-    __shared__ double red[4 * kL2Block];
+    __shared__ double red[4 * kL2Block / 64];
     const int e = blockIdx.x * kL2Block + static_cast<int>(threadIdx.x);
     double ik[4] = {0.0, 0.0, 0.0, 0.0};
     if (e < prm.ne) {
@@ -94,27 +120,17 @@ __global__ void __launch_bounds__(kL2Block) l2_kernel(L2Params prm) {
             ik[k] = det * sum;
         }
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) red[k * kL2Block + threadIdx.x] = ik[k];
-    __syncthreads();
-#pragma unroll
-    for (int w = kL2Block / 2; w > 0; w >>= 1) {
-        if (static_cast<int>(threadIdx.x) < w)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) red[k * kL2Block + threadIdx.x] += red[k * kL2Block + threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x < 4) prm.part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x * kL2Block];
+    block_sums<4, kL2Block>(ik, red, prm.part + 4 * blockIdx.x);
 }
 
 // dJ/d(eps, x0, y0) (see VGradParams).  The tangent of the vortex initial
 // state of Element.cpp:134-160 at each node, as the host's vortex_tangent
 // forms it; per element the sum runs over nodes and variables in order, the
-// block sum is a fixed-shape tree.
+// block sums are block_sums'.
 template <int P>
 __global__ void __launch_bounds__(kVGradBlock) vortex_grad_kernel(VGradParams prm) {
     constexpr int NP = (P + 1) * (P + 2) / 2;
-    __shared__ double red[3 * kVGradBlock];
+    __shared__ double red[3 * kVGradBlock / 64];
     const int e = blockIdx.x * kVGradBlock + static_cast<int>(threadIdx.x);
     double ga[3] = {0.0, 0.0, 0.0};
     if (e < prm.ne) {
@@ -158,17 +174,7 @@ __global__ void __launch_bounds__(kVGradBlock) vortex_grad_kernel(VGradParams pr
             }
         }
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) red[a * kVGradBlock + threadIdx.x] = ga[a];
-    __syncthreads();
-#pragma unroll
-    for (int w = kVGradBlock / 2; w > 0; w >>= 1) {
-        if (static_cast<int>(threadIdx.x) < w)
-#pragma unroll
-            for (int a = 0; a < 3; ++a) red[a * kVGradBlock + threadIdx.x] += red[a * kVGradBlock + threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x < 3) prm.part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x * kVGradBlock];
+    block_sums<3, kVGradBlock>(ga, red, prm.part + 3 * blockIdx.x);
 }
 
 // The vortex of Element.cpp:134-160 at (x, y, t): rho, rho u, rho v, rho E,
