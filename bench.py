@@ -562,7 +562,7 @@ def c1_figure(vt, with_cpu, order=8):
     """The reference's own small case run to the end: 8 x 8 cells of the
     10 x 10 domain (128 triangles), p = 3, 10,000 RK4 steps of
     dt = 1.414213562373e-3 (T = 10 sqrt 2) at the given integration order
-    (8: the C1 configuration; 6: the reference's shipped input), time to
+    (8: the 8 x 8 case at integration order 8; 6: the reference's shipped input), time to
     solution on the GPU and, beside it, the reference's translation units on
     the host."""
     import torch

@@ -19,7 +19,7 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 
 # name: (input-file keys, steps, rhs_at_end)
 CASES = {
-    # C1: the shipped input with integration_order=8, full period T = 10*sqrt(2)
+    # the shipped input with integration_order=8, full period T = 10*sqrt(2)
     "c1_p3_o8_8x8_full": (dict(p=3, nx=8, ny=8, integration_order=8), 10000, False),
     "c1_p3_o8_8x8_250": (dict(p=3, nx=8, ny=8, integration_order=8), 250, True),
     # the shipped input as is (integration_order=6)

@@ -117,7 +117,7 @@ def test_drivers_reproduce_reference_output(gpu, tmp_path, exe):
     path = os.path.join(HOST, exe) if exe == "vortex_transport_gpu" else os.path.join(ROOT, "oracle", "_ref", exe)
     if not os.path.exists(path):
         pytest.skip("%s not built" % exe)
-    # the C1 input: dt = 14.14213562373 / 10000 as in the fixtures, a dump every 250 steps
+    # the shipped input at integration order 8: dt = 14.14213562373 / 10000 as in the fixtures, a dump every 250 steps
     d250 = np.load(os.path.join(GOLD, "c1_p3_o8_8x8_250.npz"))
     dfull = np.load(os.path.join(GOLD, "c1_p3_o8_8x8_full.npz"))
     inp = _input(str(tmp_path), p=3, nx=8, ny=8, integration_order=8, number_time_steps=10000, write_every_steps=250)
