@@ -129,6 +129,7 @@ hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* 
     prm.ld = c->ld;
     prm.blocks_per_xcd = vt::tiles_per_xcd(c->ne, c->cfg.block);
     prm.pairs = c->pairs;
+    prm.plain_order = vti::plain_order_for(c, c->cfg.block);
     prm.dt = dt;
     prm.acc = c->acc;
     const double* in[4] = {S0, S1, S2, S3};

@@ -121,6 +121,7 @@ vt::StageParams base_params(vt_ctx* c) {
     prm.ld = c->ld;
     prm.blocks_per_xcd = vt::tiles_per_xcd(c->ne, c->cfg.block);
     prm.pairs = c->pairs;
+    prm.plain_order = vti::plain_order_for(c, c->cfg.block);
     return prm;
 }
 
@@ -335,6 +336,22 @@ int create_ctx(int n, const double* vertices, const int* neighbors, const int* t
     c->pairs = n % 2 == 0 ? 1 : 0;
     for (int e = 0; e < n && c->pairs; ++e)
         if (ns[3 * static_cast<size_t>(e) + 1] != (e ^ 1)) c->pairs = 0;
+    // Tile order per tile size: plain when at least 90% of the neighbour pairs
+    // whose tiles are more than one tile apart (the rows above and below)
+    // are a multiple of 8 tiles apart, i.e. on the same XCD in plain order.
+    for (int bit = 0; bit < 3; ++bit) {
+        const int block = 64 << bit;
+        long long far = 0, same = 0;
+        for (int e = 0; e < n; ++e)
+            for (int s = 0; s < 3; ++s) {
+                const int d = ns[3 * static_cast<size_t>(e) + s] / block - e / block;
+                if (d > 1 || d < -1) {
+                    ++far;
+                    if (d % 8 == 0) ++same;
+                }
+            }
+        if (far > 0 && same * 10 >= far * 9) c->plain_order |= 1 << bit;
+    }
     // Small-mesh path: the mesh fits one workgroup's LDS and every side's
     // neighbour faces back across the same side, so each shared side can be
     // listed once (as e | nb << 10 | s << 20 with e < nb).

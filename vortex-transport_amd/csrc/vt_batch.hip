@@ -84,6 +84,7 @@ hipError_t enqueue_stage(vt_batch* b, double dt, int mode, int stage_id, Bufs bu
             prm.stage_id = stage_id;
             prm.err_base = static_cast<int>(b->gbase[gi]);
             prm.pairs = c->pairs;
+            prm.plain_order = vti::plain_order_for(c, one.block);
             prm.blocks_per_xcd = vt::tiles_per_xcd(c->ne, one.block);
             e = hipStreamWaitEvent(c->stream, b->fork, 0);
             if (e == hipSuccess) e = vt::launch_stage(prm, c->p, c->nq, one, c->stream);

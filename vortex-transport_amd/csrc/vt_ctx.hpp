@@ -14,6 +14,7 @@
 struct vt_ctx {
     int ne = 0, ld = 0, p = 0, np = 0, nq = 0, order = 0, device = 0;
     int pairs = 0;                           // side s = 1 of every element e faces e ^ 1
+    int plain_order = 0;                     // bit i: plain tile order for tiles of 64 << i elements
     hipStream_t stream = nullptr;
     double *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr, *vtx = nullptr, *tab = nullptr;
     int* nbr = nullptr;
@@ -75,6 +76,13 @@ bool hip_ok(hipError_t e, const char* what);
 // Plane stride: element count rounded to 256 plus a 64-element skew, so the
 // 4*Np planes of one element do not all start at the same HBM channel offset.
 inline int plane_stride(int ne) { return ((ne + 255) / 256) * 256 + 64; }
+
+// Whether the stage kernels of context c run in plain tile order for tiles
+// of `block` elements (vt_ctx::plain_order, chosen at creation).
+inline int plain_order_for(const vt_ctx* c, int block) {
+    const int bit = block == 64 ? 0 : block == 128 ? 1 : block == 256 ? 2 : -1;
+    return bit >= 0 ? (c->plain_order >> bit) & 1 : 0;
+}
 
 // Largest mesh of one context: kernels address an element within a plane by
 // a 32-bit byte offset (e * 8) and the neighbour table by the int s * ld + e;

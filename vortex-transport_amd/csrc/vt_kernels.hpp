@@ -34,6 +34,7 @@ struct StageParams {
     int blocks_per_xcd;   // tiles per XCD band: ceil(ceil(ne / block) / 8)
     int err_base;         // added to the element index of an error report
     int pairs;            // 1: the neighbour across side s = 1 of every element e is e ^ 1
+    int plain_order;      // 1: workgroup b works on tile b; 0: XCD band order (stage_kernel)
 };
 
 // Launch shape.  block: elements per workgroup (128 or 256).  persistent:
