@@ -339,18 +339,21 @@ int create_ctx(int n, const double* vertices, const int* neighbors, const int* t
     // Tile order per tile size: plain when at least 90% of the neighbour pairs
     // whose tiles are more than one tile apart (the rows above and below)
     // are a multiple of 8 tiles apart, i.e. on the same XCD in plain order.
-    for (int bit = 0; bit < 3; ++bit) {
-        const int block = 64 << bit;
-        long long far = 0, same = 0;
+    {
+        long long far[3] = {0, 0, 0}, same[3] = {0, 0, 0};  // tiles of 64, 128, 256 elements
         for (int e = 0; e < n; ++e)
             for (int s = 0; s < 3; ++s) {
-                const int d = ns[3 * static_cast<size_t>(e) + s] / block - e / block;
-                if (d > 1 || d < -1) {
-                    ++far;
-                    if (d % 8 == 0) ++same;
+                const int nb = ns[3 * static_cast<size_t>(e) + s];
+                for (int bit = 0; bit < 3; ++bit) {
+                    const int d = (nb >> (6 + bit)) - (e >> (6 + bit));
+                    if (d > 1 || d < -1) {
+                        ++far[bit];
+                        if ((d & 7) == 0) ++same[bit];
+                    }
                 }
             }
-        if (far > 0 && same * 10 >= far * 9) c->plain_order |= 1 << bit;
+        for (int bit = 0; bit < 3; ++bit)
+            if (far[bit] > 0 && same[bit] * 10 >= far[bit] * 9) c->plain_order |= 1 << bit;
     }
     // Small-mesh path: the mesh fits one workgroup's LDS and every side's
     // neighbour faces back across the same side, so each shared side can be
