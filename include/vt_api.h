@@ -177,9 +177,12 @@ int vt_l2_state_error(vt_ctx *ctx, const double *U_ref, double err[4]);
 
 /* Launch shape of the stage kernels: block = elements per workgroup (64, the
  * default, 128 or 256); persistent != 0 runs 8 * slots_per_xcd resident workgroups that loop
- * over the element tiles instead of one workgroup per tile.  Results do not
- * depend on it.  No reference counterpart (the reference's OpenMP loop,
- * Evolve.cpp:7-9, has no launch shape). */
+ * over the element tiles instead of one workgroup per tile.  The order in which
+ * the workgroups take the tiles (one contiguous band per XCD, or tile b for
+ * workgroup b where the mesh's rows hold a multiple of 8 tiles) is chosen by
+ * vt_create from the connectivity.  Results do not depend on either.  No
+ * reference counterpart (the reference's OpenMP loop, Evolve.cpp:7-9, has no
+ * launch shape). */
 int vt_set_launch(vt_ctx *ctx, int block, int persistent, int slots_per_xcd);
 int vt_get_launch(const vt_ctx *ctx, int *cfg /* [3] block, persistent, slots_per_xcd */);
 /* Small-mesh path of vt_step (the reference's own 8 x 8-cell case): a mesh
