@@ -77,6 +77,7 @@ def parse_args():
     ap.add_argument("--adjoint-steps", type=int, default=1000)
     ap.add_argument("--no-batched-adjoint", action="store_true")
     ap.add_argument("--batched-adjoint-steps", type=int, default=500)
+    ap.add_argument("--batched-adjoint-every", type=int, default=16, help="checkpoint interval of the ensemble adjoint")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-shard", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
@@ -761,7 +762,7 @@ def main():
     if not a.no_adjoint:
         line["adjoint"] = adjoint_figure(vt, a.adjoint_steps)
     if not a.no_batched_adjoint:
-        line["batched_adjoint"] = batched_adjoint_figure(vt, a.batched_adjoint_steps)
+        line["batched_adjoint"] = batched_adjoint_figure(vt, a.batched_adjoint_steps, every=a.batched_adjoint_every)
     if not a.no_large:
         line["large_mesh"] = large_mesh_figure(vt, a.large_steps)
     if not a.no_shard:
