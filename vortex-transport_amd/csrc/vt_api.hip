@@ -595,7 +595,7 @@ int vt_get_flux(vt_ctx* c, double* F) {
     std::vector<double> U(static_cast<size_t>(c->ne) * c->np * 4);
     int rc = read_planes(c, c->Un, U.data());
     if (rc) return rc;
-    for (size_t i = 0; i < static_cast<size_t>(c->ne) * c->np; ++i) vt::nodal_flux(&U[4 * i], &F[8 * i]);
+    vt::nodal_flux_n(static_cast<size_t>(c->ne) * c->np, U.data(), F);
     return VT_OK;
 }
 

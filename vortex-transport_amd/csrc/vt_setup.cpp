@@ -306,6 +306,12 @@ void nodal_flux(const double* U, double* F) {
     F[4] = rho * v; F[5] = rho * u * v; F[6] = rho * std::pow(v, 2) + p; F[7] = rho * v * H;
 }
 
+void nodal_flux_n(size_t n, const double* U, double* F) {
+    const long long nn = static_cast<long long>(n);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < nn; ++i) nodal_flux(U + 4 * i, F + 8 * i);
+}
+
 bool initial_state(int ne, const double* V, int p, int order, int init_type, double epsilon, double x0, double y0,
                    double t, double* U) {
     const int np = num_nodes(p);

@@ -6,6 +6,7 @@
 // the same bits as the reference computes on the host (expression order kept
 // where the reference fixes it; files cited per function).
 #pragma once
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
@@ -79,6 +80,8 @@ void planes_to_aos(int n, int np, int ld, int e0, const double* planes, double* 
 void density_to_planes(int n, int np, int ld, const double* rho, double* planes);
 // Nodal flux from a state, the refresh of Timestepping.cpp:59-76.
 void nodal_flux(const double* U /*[4]*/, double* F /*[2][4]*/);
+// nodal_flux of n nodes, U [n][4] -> F [n][2][4], on all host threads.
+void nodal_flux_n(std::size_t n, const double* U, double* F);
 
 // Edge node lists: the P+1 nodes of the reference triangle lying on each side.
 // Basis functions of all other nodes vanish on that side (their restriction is

@@ -116,6 +116,7 @@ Binding& bind(Element* el, Evolve_element* ev, int n) {
 
 void upload(Binding& b, const Element* el) {
     std::vector<double> U(static_cast<size_t>(b.n) * b.np * 4);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < b.n; ++i)
         for (int j = 0; j < b.np; ++j)
             for (int k = 0; k < 4; ++k) U[(static_cast<size_t>(i) * b.np + j) * 4 + k] = el[i].hidrodynamics_vector_U[j][k];
@@ -129,6 +130,7 @@ void download(Binding& b, Element* el) {
     std::vector<double> U(static_cast<size_t>(b.n) * b.np * 4);
     int rc = vt_get_state(b.ctx, U.data());
     if (rc) die_hip("vt_get_state", rc);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < b.n; ++i)
         for (int j = 0; j < b.np; ++j) {
             const double* u = &U[(static_cast<size_t>(i) * b.np + j) * 4];
