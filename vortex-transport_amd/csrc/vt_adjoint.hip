@@ -107,6 +107,7 @@ vt::AdjParams adj_params(vt_ctx* c, int mode) {
     a.ld = c->ld;
     a.mode = mode;
     a.blocks_per_xcd = ((c->ne + vt::kAdjBlock - 1) / vt::kAdjBlock + 7) / 8;
+    a.pairs = c->pairs;
     return a;
 }
 

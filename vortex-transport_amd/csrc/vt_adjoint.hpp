@@ -30,12 +30,14 @@
 
 namespace vt {
 
-// (dF/dUL)^T bF for the Roe flux of Numericalflux.cpp:31-156, differentiated on
-// the branches the forward takes (entropy fix, |.|); reciprocals as in roe_flux.
-// The cotangent comes halved, hF = bF / 2: every path from F to the state
-// carries the flux's factor 1/2, which the caller folds into its weights.
-__device__ __forceinline__ void roe_flux_vjp_left(const double (&UL)[4], const double (&UR)[4], double nx,
-                                                  double ny, const double (&hF)[4], double (&bUL)[4]) {
+// (dF/dUL)^T bF, and with RIGHT also (dF/dUR)^T bF, for the Roe flux of
+// Numericalflux.cpp:31-156, differentiated on the branches the forward takes
+// (entropy fix, |.|); reciprocals as in roe_flux.  The cotangent comes halved,
+// hF = bF / 2: every path from F to the state carries the flux's factor 1/2,
+// which the caller folds into its weights.
+template <bool RIGHT>
+__device__ __forceinline__ void roe_flux_vjp(const double (&UL)[4], const double (&UR)[4], double nx, double ny,
+                                             const double (&hF)[4], double (&bUL)[4], double (&bUR)[4]) {
     constexpr double gmi = 0.4;
     // forward sweep
     const double rL = UL[0], irL = fast_rcp(rL);
@@ -135,6 +137,36 @@ __device__ __forceinline__ void roe_flux_vjp_left(const double (&UL)[4], const d
     bUL[1] = bU1 - bdu1;
     bUL[2] = bU2 - bdu2;
     bUL[3] = bU3 - bdu3;
+    if (RIGHT) {
+        // Roe averages, right-state dependences: d(ui)/d(uR) = di d1, and
+        // di = sqrt(rR / rL) gives d(di)/d(rR) = di / (2 rR).
+        const double dd = di * d1;
+        const double buR = bui * dd, bvR = bvi * dd, bHR = bHi * dd;
+        double brR = 0.5 * bdi * di * irR;
+        // right-state flux F_R = (rR unR, UR1 unR + pR nx, UR2 unR + pR ny, rHR unR), weight 1/2
+        const double unR = uR * nx + vR * ny, rHR = UR[3] + pR;
+        brR += h0 * unR;
+        const double bunR = h0 * rR + h1 * UR[1] + h2 * UR[2] + h3 * rHR;
+        double bV1 = h1 * unR, bV2 = h2 * unR;
+        double bpR = h1 * nx + h2 * ny;
+        const double brHR = h3 * unR + bHR * irR;
+        brR -= bHR * HR * irR;
+        double bV3 = brHR;
+        bpR += brHR;
+        const double buRt = buR + bunR * nx, bvRt = bvR + bunR * ny;
+        bV3 += gmi * bpR;
+        bV1 -= gmi * uR * bpR;
+        bV2 -= gmi * vR * bpR;
+        brR += gmi * 0.5 * (uR * uR + vR * vR) * bpR;
+        bV1 += buRt * irR;
+        brR -= buRt * uR * irR;
+        bV2 += bvRt * irR;
+        brR -= bvRt * vR * irR;
+        bUR[0] = brR + bdu0;
+        bUR[1] = bV1 + bdu1;
+        bUR[2] = bV2 + bdu2;
+        bUR[3] = bV3 + bdu3;
+    }
 }
 
 // (dFx/dU)^T bFx + (dFy/dU)^T bFy of the nodal Euler flux (Timestepping.cpp:59-76),
@@ -279,12 +311,20 @@ __global__ void __launch_bounds__(kAdjBlock, 1) adjoint_stage_kernel(AdjParams p
                 un[m][k] = plane_at(U, v, ld8, onb[s]);
                 uo[m][k] = plane_at(U, v, ld8, oe);
             }
+        // On a paired mesh side s = 1 is the diagonal of the element's cell,
+        // shared with lane ^ 1 (as in the forward stage kernel): each of the two
+        // sweeps the points q < ceil(NQ/2) of its own orientation and, except at
+        // a middle point, also returns the derivative with respect to the other
+        // element's trace, which is the other element's own derivative at its
+        // point NQ-1-q (d and F both change sign there), passed by a lane swap.
+        const bool half = (s == 1) && prm.pairs;
+        const int nqe = half ? (NQ + 1) / 2 : NQ;
 #pragma unroll 1
-        for (int q = 0; q < NQ; ++q) {
+        for (int q = 0; q < nqe; ++q) {
             // d(q) = len (c_e(q) - c_nb(q~)) / 2, half the cotangent of this
             // element's flux minus that of the neighbour's flux at the same point.
             const int qn = NQ - 1 - q;
-            double UL[4], UR[4], d[4], bUL[4];
+            double UL[4], UR[4], d[4], bUL[4], bUR[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 double a = 0.0, c = 0.0, ce = 0.0, cn = 0.0;
@@ -299,7 +339,18 @@ __global__ void __launch_bounds__(kAdjBlock, 1) adjoint_stage_kernel(AdjParams p
                 UR[k] = c;
                 d[k] = hlen * (ce - cn);
             }
-            roe_flux_vjp_left(UL, UR, nxv, nyv, d, bUL);
+            if (half && q != qn) {
+                roe_flux_vjp<true>(UL, UR, nxv, nyv, d, bUL, bUR);
+                double bP[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) bP[k] = lane_pair_swap(bUR[k]);
+#pragma unroll
+                for (int m = 0; m < NE; ++m)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) g[edge_node<P>(s, m)][k] -= tp[qn * NE + m] * bP[k];
+            } else {
+                roe_flux_vjp<false>(UL, UR, nxv, nyv, d, bUL, bUR);
+            }
 #pragma unroll
             for (int m = 0; m < NE; ++m)
 #pragma unroll

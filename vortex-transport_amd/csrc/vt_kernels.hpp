@@ -79,6 +79,7 @@ struct AdjParams {
     const double* tab;
     double dt, scale;
     int ne, ld, mode, blocks_per_xcd;
+    int pairs;            // 1: the neighbour across side s = 1 of every element e is e ^ 1
 };
 
 // Objective of the adjoint, J = sum_e detJ_e drho_e^T M_ref drho_e with
