@@ -58,6 +58,38 @@ def test_rhs_vjp_in_block_and_out_of_block_neighbours(vt, oracle, gpu):
     assert rel(s.rhs_vjp(w), o.rhs_vjp(w)) < 1e-11
 
 
+@pytest.mark.parametrize("p,order", [(1, 8), (3, 8), (2, 6)])
+def test_rhs_vjp_unpaired_numbering(vt, oracle, gpu, p, order):
+    # Element e renumbered (e + 1) mod n: the two triangles of a cell are then
+    # not on lanes l, l ^ 1, so every side's reverse sweeps run in full instead
+    # of the cell diagonal's being shared between the two lanes.  Both forms
+    # agree with the oracle and, relabelled, with each other.
+    mesh = vt.generate_mesh(nx=12, ny=10, perturb_grid=1.0)
+    n = mesh.num_elements
+    pi = (np.arange(n) + 1) % n
+    v = np.empty_like(mesh.element_coordinates)
+    b = np.empty_like(mesh.elements_at_boundary)
+    t = np.empty_like(mesh.element_type)
+    v[pi] = mesh.element_coordinates
+    b[pi] = pi[mesh.elements_at_boundary]
+    t[pi] = mesh.element_type
+    m2 = vt.Mesh(v, b, t)
+    a = vt.Solver(mesh, p, order)
+    c = vt.Solver(m2, p, order)
+    a.initialize_hydrodinamics(0, 0.35, 0.4, -0.2)
+    c.set_state(a.state()[np.argsort(pi)])
+    assert np.array_equal(c.state()[pi], a.state())
+    w = np.random.default_rng(11 + p).standard_normal((n, a.np, 4))
+    wc = w[np.argsort(pi)]
+    ga, gc = a.rhs_vjp(w), c.rhs_vjp(wc)
+    o = oracle.Solver(m2.element_coordinates, m2.elements_at_boundary, m2.element_type, p, order)
+    o.set_state(c.state())
+    assert rel(gc, o.rhs_vjp(wc)) < 1e-11
+    assert rel(gc[pi], ga) < 1e-13, rel(gc[pi], ga)
+    a.close()
+    c.close()
+
+
 @pytest.mark.parametrize("nsteps,every", [(37, 8), (16, 16), (5, 1), (9, 64)])
 def test_rk4_adjoint_matches_oracle(vt, oracle, gpu, nsteps, every):
     dt = 1e-3
