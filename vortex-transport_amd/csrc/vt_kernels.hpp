@@ -154,8 +154,9 @@ struct VortexParams {
 // same side index on both), ordered by s: faces[fbeg[s] .. fbeg[s + 1]).
 // Un holds the state before and after the launch; the first failure goes to
 // err with stage ids counted from 0 in the launch.
-constexpr int kSmallThreads = 512;
-constexpr int kSmallMaxElements = kSmallThreads / 4;  // one thread per (variable, element)
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallItems = 512;                     // role B threads, one per (variable, element)
+constexpr int kSmallMaxElements = kSmallItems / 4;
 
 struct SmallParams {
     double* Un;           // state planes [4*NP][ld]

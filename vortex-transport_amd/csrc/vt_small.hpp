@@ -5,18 +5,24 @@
 // A small mesh (the reference's own 8 x 8-cell case has 128 elements) leaves
 // the one-thread-per-element stage kernel with two workgroups on a 256-CU
 // device, each thread running its element's whole serial chain and every
-// stage paying a launch.  Here one workgroup of kSmallThreads threads (8
-// wavefronts on one CU) holds the stage state in LDS across all the steps of
-// a launch and spreads each stage over the threads in two phases:
-//   A  one thread per (shared side, Gauss point): the two traces and one Roe
-//      flux evaluation, stored for both elements (F for the element whose
-//      orientation was used, -F at the mirrored point for its neighbour, by
-//      the flux's antisymmetry F(a, b, n) = -F(b, a, -n)), so every physical
-//      flux point is evaluated once instead of once per side;
-//   B  one thread per (variable k, element e): the volume term of variable k
-//      (Sxi G^xi + Seta G^eta from the nodal flux), the lift of the stored
+// stage paying a launch.  Here one workgroup of kSmallThreads threads (16
+// wavefronts on one CU, four per SIMD) holds the stage state in LDS across all
+// the steps of a launch and spreads each stage over the threads in two roles
+// that meet at the stage's two barriers:
+//   A  threads [kSmallItems, kSmallThreads), one (shared side, Gauss point) at
+//      a time: the two traces and one Roe flux evaluation, stored for both
+//      elements (F for the element whose orientation was used, -F at the
+//      mirrored point for its neighbour, by the flux's antisymmetry
+//      F(a, b, n) = -F(b, a, -n)), so every physical flux point is evaluated
+//      once instead of once per side;
+//   B  threads [0, kSmallItems), one (variable k, element e) each: the volume
+//      term of variable k (Sxi G^xi + Seta G^eta from the nodal flux) beside
+//      the A threads' fluxes, then after the barrier the lift of the stored
 //      fluxes, Minv_ref / detJ and the RK update.  U_n and the running sum
 //      k1 + 2 k2 + 2 k3 stay in this thread's registers for the whole launch.
+// The roles are uniform over a wavefront and each has its own loop over the
+// stages, so only one role's code runs on a wavefront and each role's
+// registers are its own (at most 128 per thread for four wavefronts per SIMD).
 // The stage update follows Timestepping.cpp:85-305 operation by operation:
 // stage states U_n + k_s / 2, U_n + k_3, and U_n + (1/6)(k1 + 2 k2 + 2 k3 + k4).
 #pragma once
@@ -94,63 +100,6 @@ __device__ __forceinline__ void small_side_flux(const double* S, double* Fs, con
     }
 }
 
-// Phase A for two items at once (points of this thread's two rounds), in
-// one instruction stream so that their two flux evaluations interleave.  The
-// side index of each is wavefront-uniform; an item with ok false (past the
-// end of its side group) is evaluated on a valid side and stores nothing.
-// The last round keeps small_side_flux with its compile-time side: one
-// function over both forms (item count and side as runtime values) measured
-// 14% slower per stage (256 VGPRs with spills against 246).
-template <int P, int NQ, int ns>
-__device__ __forceinline__ void small_side_flux2(const double* S, double* Fs, const double* Nf, const double* Tw,
-                                                 const int* Fc, unsigned long long* key, int nfp, const int (&sd)[2],
-                                                 const int (&f)[2], const int (&q)[2], const bool (&ok)[2], int st) {
-    constexpr int NP = (P + 1) * (P + 2) / 2;
-    constexpr int NE = P + 1;
-    double UL[2][4], UR[2][4], F[2][4];
-    int e[2], nb[2], bits[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int fc = Fc[f[t]];
-        e[t] = fc & 1023;
-        nb[t] = (fc >> 10) & 1023;
-        const double* tp = Tw + (sd[t] * NQ + q[t]) * NE;
-        const double* tm = tp + 3 * NQ * NE;
-        int en[NE];
-#pragma unroll
-        for (int m = 0; m < NE; ++m) en[m] = sd[t] == 0 ? edge_node<P>(0, m) : sd[t] == 1 ? edge_node<P>(1, m) : edge_node<P>(2, m);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            double a = 0.0, b = 0.0;
-#pragma unroll
-            for (int m = 0; m < NE; ++m) {
-                a += tp[m] * S[(k * NP + en[m]) * ns + e[t]];
-                b += tm[m] * S[(k * NP + en[m]) * ns + nb[t]];
-            }
-            UL[t][k] = a;
-            UR[t][k] = b;
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < 2; ++t) bits[t] = roe_flux_bits(UL[t], UR[t], Nf[f[t]], Nf[nfp + f[t]], F[t]);
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        if (!ok[t]) continue;
-        const int qm = NQ - 1 - q[t];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            Fs[((sd[t] * NQ + q[t]) * 4 + k) * ns + e[t]] = F[t][k];
-            Fs[((sd[t] * NQ + qm) * 4 + k) * ns + nb[t]] = -F[t][k];
-        }
-        if (bits[t]) {
-            const unsigned long long stage = static_cast<unsigned long long>(st) << 40;
-            const int ce = flux_code(bits[t]), cn = flux_code(swap_sides(bits[t]));
-            if (ce) atomicMin(key, stage | ((((static_cast<unsigned long long>(e[t]) * 16 + q[t]) * 4 + sd[t]) * 4) + ce));
-            if (cn) atomicMin(key, stage | ((((static_cast<unsigned long long>(nb[t]) * 16 + qm) * 4 + sd[t]) * 4) + cn));
-        }
-    }
-}
-
 // Volume term of variable K at element e: r_i += Sxi_ij G^xi_j + Seta_ij G^eta_j
 // with G = adj(J) F_K(U_j), the nodal flux as the stage kernel forms it.
 template <int P, int K, int ns>
@@ -192,6 +141,7 @@ __device__ __forceinline__ double rounded_product(double a, double b) {
 
 template <int P, int NQ, int NEP>
 __global__ void __launch_bounds__(kSmallThreads, 1) small_run_kernel(SmallParams prm) {
+    constexpr int NT = kSmallThreads;
     constexpr int NP = (P + 1) * (P + 2) / 2;
     constexpr int NE = P + 1;
     extern __shared__ double lds[];
@@ -217,19 +167,19 @@ __global__ void __launch_bounds__(kSmallThreads, 1) small_run_kernel(SmallParams
     // Phase B item of this thread: variable kb of element eb (a wavefront
     // covers 64 elements of one variable, so kb is wavefront-uniform).
     const int kb = __builtin_amdgcn_readfirstlane(tid / nep), eb = tid - kb * nep;
-    const bool bact = kb < 4 && eb < prm.ne;
+    const bool bact = tid < kSmallItems && kb < 4 && eb < prm.ne;
     double un[NP], sk[NP];  // U_n and k1 + 2 k2 + 2 k3 of (kb, eb)
     double a00 = 0, a01 = 0, a10 = 0, a11 = 0, idet = 0, len[3] = {0, 0, 0};
     if (tid == 0) *key = kNoError;
-    for (int i = tid; i < 4 * NP * nep; i += kSmallThreads) {
+    for (int i = tid; i < 4 * NP * nep; i += NT) {
         const int v = i / nep, e = i - v * nep;
         S[v * ns + e] = e < prm.ne ? plane_at(prm.Un, v, ld8, static_cast<unsigned>(e) * 8u) : 0.0;
     }
-    for (int i = tid; i < 3 * NQ * NE; i += kSmallThreads) {
+    for (int i = tid; i < 3 * NQ * NE; i += NT) {
         Tw[i] = prm.tab[tab_tp(P) + i];
         Tw[3 * NQ * NE + i] = prm.tab[tab_tm(P, NQ) + i];
     }
-    for (int f = tid; f < prm.nfaces; f += kSmallThreads) {
+    for (int f = tid; f < prm.nfaces; f += NT) {
         const int fc = prm.faces[f];
         const int e = fc & 1023, s = fc >> 20, s1 = s == 2 ? 0 : s + 1;
         const unsigned oe = static_cast<unsigned>(e) * 8u;
@@ -263,113 +213,107 @@ __global__ void __launch_bounds__(kSmallThreads, 1) small_run_kernel(SmallParams
     __syncthreads();
 
     const int nstages = prm.method == 1 ? 4 * prm.nsteps : prm.nsteps;
-    for (int st = 0; st < nstages; ++st) {
-        const int sub = prm.method == 1 ? (st & 3) : 0;  // RK4 stage within the step
-        // Phase A: one flux evaluation per listed side and Gauss point.  The
-        // sides are listed by side index s, each group's items padded to whole
-        // wavefronts, so s is uniform over a wavefront.
-        // Two rounds at a time: while the wavefront's item of the next round
-        // exists, both are evaluated together (small_side_flux2).
-        int it0 = tid - (tid & 63);
-        for (; it0 + kSmallThreads < ib[3]; it0 += 2 * kSmallThreads) {
-            int sd[2], f[2], q[2];
-            bool ok[2];
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int w0 = it0 + t * kSmallThreads;
-                sd[t] = __builtin_amdgcn_readfirstlane((w0 >= ib[1]) + (w0 >= ib[2]));
-                const int li = w0 + (tid & 63) - ib[sd[t]];
-                const int cnt = (prm.fbeg[sd[t] + 1] - prm.fbeg[sd[t]]) * NQ;
-                ok[t] = li < cnt;
-                const int lc = ok[t] ? li : 0;
-                f[t] = prm.fbeg[sd[t]] + lc / NQ;
-                q[t] = lc - (lc / NQ) * NQ;
+    if (__builtin_amdgcn_readfirstlane(tid) >= kSmallItems) {
+        // Role A: per stage one flux evaluation per listed side and Gauss
+        // point, then the same two barriers as role B and the same stop after
+        // the first failing stage.  The sides are listed by side index s,
+        // each group's items padded to whole wavefronts, so s is uniform over
+        // a wavefront; one item per thread and round.
+        constexpr int NA = kSmallThreads - kSmallItems;
+        const int w0 = __builtin_amdgcn_readfirstlane(tid - kSmallItems) & ~63;
+        for (int st = 0; st < nstages; ++st) {
+            for (int it0 = w0; it0 < ib[3]; it0 += NA) {
+                const int it = it0 + (tid & 63);
+                const int sd = __builtin_amdgcn_readfirstlane((it0 >= ib[1]) + (it0 >= ib[2]));
+                const int li = it - ib[sd];
+                if (li >= (prm.fbeg[sd + 1] - prm.fbeg[sd]) * NQ) continue;
+                const int f = prm.fbeg[sd] + li / NQ, q = li - (li / NQ) * NQ;
+                if (sd == 0) small_side_flux<P, NQ, 0, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, f, q, st);
+                else if (sd == 1) small_side_flux<P, NQ, 1, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, f, q, st);
+                else small_side_flux<P, NQ, 2, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, f, q, st);
             }
-            small_side_flux2<P, NQ, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, sd, f, q, ok, st);
+            __syncthreads();
+            if (*key != kNoError) break;
+            __syncthreads();
         }
-        for (; it0 < ib[3]; it0 += kSmallThreads) {
-            const int it = it0 + (tid & 63);
-            const int sd = __builtin_amdgcn_readfirstlane((it0 >= ib[1]) + (it0 >= ib[2]));
-            const int li = it - ib[sd];
-            if (li >= (prm.fbeg[sd + 1] - prm.fbeg[sd]) * NQ) continue;
-            const int f = prm.fbeg[sd] + li / NQ, q = li - (li / NQ) * NQ;
-            if (sd == 0) small_side_flux<P, NQ, 0, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, f, q, st);
-            else if (sd == 1) small_side_flux<P, NQ, 1, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, f, q, st);
-            else small_side_flux<P, NQ, 2, ns>(S, Fs, Nf, Tw, Fc, key, L.nfp, f, q, st);
-        }
-        // The volume term reads only the stage state: formed before the
-        // barrier, beside the flux evaluations.
-        double r[NP];
+    } else {
+        // Role B: the volume term beside role A's fluxes, then phase B.
+        for (int st = 0; st < nstages; ++st) {
+            const int sub = prm.method == 1 ? (st & 3) : 0;  // RK4 stage within the step
+            // The volume term reads only the stage state: formed before the
+            // barrier, beside role A's flux evaluations.
+            double r[NP];
 #pragma unroll
-        for (int i = 0; i < NP; ++i) r[i] = 0.0;
-        if (bact) {
-            switch (kb) {
-            case 0: small_volume<P, 0, ns>(S, eb, a00, a01, a10, a11, T, r); break;
-            case 1: small_volume<P, 1, ns>(S, eb, a00, a01, a10, a11, T, r); break;
-            case 2: small_volume<P, 2, ns>(S, eb, a00, a01, a10, a11, T, r); break;
-            default: small_volume<P, 3, ns>(S, eb, a00, a01, a10, a11, T, r); break;
+            for (int i = 0; i < NP; ++i) r[i] = 0.0;
+            if (bact) {
+                switch (kb) {
+                case 0: small_volume<P, 0, ns>(S, eb, a00, a01, a10, a11, T, r); break;
+                case 1: small_volume<P, 1, ns>(S, eb, a00, a01, a10, a11, T, r); break;
+                case 2: small_volume<P, 2, ns>(S, eb, a00, a01, a10, a11, T, r); break;
+                default: small_volume<P, 3, ns>(S, eb, a00, a01, a10, a11, T, r); break;
+                }
             }
-        }
-        __syncthreads();
-        if (*key != kNoError) break;  // the reference stops at its first failure
+            __syncthreads();
+            if (*key != kNoError) break;  // the reference stops at its first failure
 
-        // Phase B: the lift of the stored fluxes, Minv_ref / detJ, the update.
-        double unew[NP];
-        if (bact) {
+            // Phase B: the lift of the stored fluxes, Minv_ref / detJ, the update.
+            double unew[NP];
+            if (bact) {
 #pragma unroll
-            for (int s = 0; s < 3; ++s) {
-                const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;
+                for (int s = 0; s < 3; ++s) {
+                    const ctab_t tl = T + tab_lift(P, NQ) + s * NQ * NE;
 #pragma unroll
-                for (int q = 0; q < NQ; ++q) {
-                    // len * F once per point, then one FMA per edge node (the
-                    // reference's len * w * phi * F regrouped at the ulp level).
-                    const double lF = len[s] * Fs[((s * NQ + q) * 4 + kb) * ns + eb];
+                    for (int q = 0; q < NQ; ++q) {
+                        // len * F once per point, then one FMA per edge node (the
+                        // reference's len * w * phi * F regrouped at the ulp level).
+                        const double lF = len[s] * Fs[((s * NQ + q) * 4 + kb) * ns + eb];
 #pragma unroll
-                    for (int m = 0; m < NE; ++m) r[edge_node<P>(s, m)] -= tl[q * NE + m] * lF;
+                        for (int m = 0; m < NE; ++m) r[edge_node<P>(s, m)] -= tl[q * NE + m] * lF;
+                    }
+                }
+                const double dt = prm.dt;
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    double d = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NP; ++j) d += T[tab_minv(P) + i * NP + j] * r[j];
+                    const double kk = dt * (d * idet);
+                    // The reference's update expressions, each operation rounded
+                    // on its own (Timestepping.cpp:137,179,221,279, forward_euler
+                    // :50); k/2 and 2k are exact, so only U_n + (1/6)(...) needs
+                    // its multiply kept apart from its add.
+                    if (prm.method == 0) {
+                        unew[i] = S[(kb * NP + i) * ns + eb] + kk;
+                    } else if (sub == 0) {
+                        sk[i] = kk;
+                        unew[i] = un[i] + kk / 2.0;
+                    } else if (sub == 1) {
+                        sk[i] = sk[i] + 2.0 * kk;
+                        unew[i] = un[i] + kk / 2.0;
+                    } else if (sub == 2) {
+                        sk[i] = sk[i] + 2.0 * kk;
+                        unew[i] = un[i] + kk;
+                    } else {
+                        un[i] = un[i] + rounded_product(1.0 / 6.0, sk[i] + kk);
+                        unew[i] = un[i];
+                    }
                 }
             }
-            const double dt = prm.dt;
+            // No barrier before the write: every read of another element's state
+            // (phase A's traces, the volume term) ended at the barrier above, and
+            // phase B reads only this thread's own entries of S.  The barrier after
+            // it orders the write before the next stage's reads, and this stage's
+            // reads of Fs before the next stage's phase A overwrites it.
+            if (bact) {
 #pragma unroll
-            for (int i = 0; i < NP; ++i) {
-                double d = 0.0;
-#pragma unroll
-                for (int j = 0; j < NP; ++j) d += T[tab_minv(P) + i * NP + j] * r[j];
-                const double kk = dt * (d * idet);
-                // The reference's update expressions, each operation rounded
-                // on its own (Timestepping.cpp:137,179,221,279, forward_euler
-                // :50); k/2 and 2k are exact, so only U_n + (1/6)(...) needs
-                // its multiply kept apart from its add.
-                if (prm.method == 0) {
-                    unew[i] = S[(kb * NP + i) * ns + eb] + kk;
-                } else if (sub == 0) {
-                    sk[i] = kk;
-                    unew[i] = un[i] + kk / 2.0;
-                } else if (sub == 1) {
-                    sk[i] = sk[i] + 2.0 * kk;
-                    unew[i] = un[i] + kk / 2.0;
-                } else if (sub == 2) {
-                    sk[i] = sk[i] + 2.0 * kk;
-                    unew[i] = un[i] + kk;
-                } else {
-                    un[i] = un[i] + rounded_product(1.0 / 6.0, sk[i] + kk);
-                    unew[i] = un[i];
-                }
+                for (int i = 0; i < NP; ++i) S[(kb * NP + i) * ns + eb] = unew[i];
             }
+            __syncthreads();
         }
-        // No barrier before the write: every read of another element's state
-        // (phase A's traces, the volume term) ended at the barrier above, and
-        // phase B reads only this thread's own entries of S.  The barrier after
-        // it orders the write before the next stage's reads, and this stage's
-        // reads of Fs before the next stage's phase A overwrites it.
-        if (bact) {
-#pragma unroll
-            for (int i = 0; i < NP; ++i) S[(kb * NP + i) * ns + eb] = unew[i];
-        }
-        __syncthreads();
     }
 
     // The state at the end of the launch (or of the failing stage) back to U_n.
-    for (int i = tid; i < 4 * NP * nep; i += kSmallThreads) {
+    for (int i = tid; i < 4 * NP * nep; i += NT) {
         const int v = i / nep, e = i - v * nep;
         if (e < prm.ne) plane_at(prm.Un, v, ld8, static_cast<unsigned>(e) * 8u) = S[v * ns + e];
     }
