@@ -373,12 +373,19 @@ def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
     t0 = time.perf_counter()
     b.rk4(dt, steps)
     tf = time.perf_counter() - t0
-    for i in range(b.n):
-        b.set_state(i, U0[i])
-    b.time = 0.0
-    t0 = time.perf_counter()
-    J, grad = b.adjoint_vortex(dt, steps, None, eps, x0, y0, every=every)
-    ta = time.perf_counter() - t0
+    # Two calls: the first allocates the recording (checkpoints and segment
+    # stage states, ~200 GB), which the batch keeps for the next call; the
+    # second is the steady state of repeated adjoint solves.
+    tas, res = [], []
+    for _ in range(2):
+        for i in range(b.n):
+            b.set_state(i, U0[i])
+        b.time = 0.0
+        t0 = time.perf_counter()
+        res.append(b.adjoint_vortex(dt, steps, None, eps, x0, y0, every=every))
+        tas.append(time.perf_counter() - t0)
+    (J, grad), (J2, grad2) = res
+    ta = tas[1]
     ne = sum(b.ne)
     slot_bytes = ne * 10 * 4 * 8
     slots = -(-steps // every) + 4 * every + 8
@@ -387,6 +394,9 @@ def batched_adjoint_figure(vt, steps, dt=5e-5, every=16):
                         "integration_order 8, %d RK4 steps of dt %g, checkpoints every %d steps" % (steps, dt, every),
             "elements": ne, "steps": steps, "forward_s": tf, "forward_ms_per_step": tf / steps * 1e3,
             "forward_plus_adjoint_s": ta, "ratio": ta / tf, "launches_per_stage": 1,
+            "first_call": {"forward_plus_adjoint_s": tas[0], "ratio": tas[0] / tf,
+                           "why": "allocates the recording that the second (timed) call re-uses"},
+            "repeat_same_bits": bool(np.array_equal(J, J2) and np.array_equal(grad, grad2)),
             "parameter_gradient": "dJ_i/d(eps_i, x0_i, y0_i) contracted on the device",
             "device_state_gb": slots * slot_bytes / 1e9, "J_sum": float(np.sum(J)),
             "setup_s": {"initial_states_host": c1 - c0, "targets_host": c2 - c1,

@@ -219,6 +219,12 @@ int vt_adjoint_recording(const vt_ctx *ctx, unsigned long long *id);
  * checkpoints; 4*every + 1 states of device memory.  The current state is
  * not changed. */
 int vt_adjoint_backward(vt_ctx *ctx, const double *lamT, double *lam0);
+/* Frees the context's adjoint memory (checkpoints, segment stage states,
+ * adjoint planes, objective buffers), which it otherwise keeps for re-use
+ * until vt_destroy; the recording is gone (vt_adjoint_recording gives 0 and
+ * vt_adjoint_backward returns VT_ERR_ARG until the next vt_adjoint_forward).
+ * No reference counterpart. */
+int vt_adjoint_release(vt_ctx *ctx);
 /* on != 0 (the default): the reverse sweep recomputes the next segment's
  * stage states on a second stream while the adjoint kernels sweep the
  * current one, in a second set of 4*every states allocated when it fits
@@ -303,10 +309,15 @@ int vt_batch_mass_l2_objective(vt_batch *b, const double *rho_target, double *J,
  * (the state advances to U_T as with vt_batch_step), the objectives J [n_inst]
  * at U_T, and the reverse sweep: dJdU0 = dJ_i/dU0_i.  Device memory per
  * (p, order) group of E elements: (ceil(nsteps/every) + 4 every + 8) states
- * of E * Np * 32 bytes, held only while the group is processed.
+ * of E * Np * 32 bytes (4 every more when the overlapped sweep fits), one
+ * group's at a time: a group's is released before another group records, and
+ * the last group's is kept for re-use by the next call until
+ * vt_batch_adjoint_release or vt_batch_destroy.
  * VT_ERR_NONPHYSICAL as vt_batch_step. */
 int vt_batch_adjoint(vt_batch *b, double dt, int nsteps, int every, const double *rho_target, double *J,
                      double *dJdU0);
+/* Frees the adjoint memory vt_batch_adjoint keeps for re-use. */
+int vt_batch_adjoint_release(vt_batch *b);
 /* grad [n_inst][3] = dJ_i/d(epsilon_i, x0_i, y0_i) of type 0 vortex initial
  * states set at time t, from dJdU0 of vt_batch_adjoint (the tangent of
  * vt_vortex_tangent contracted with dJ_i/dU0_i). */

@@ -90,6 +90,55 @@ def test_batch_adjoint_equals_single_instances(vt, gpu):
     assert np.array_equal(J, J2) and all(np.array_equal(x, y) for x, y in zip(lam, lam2))
 
 
+def test_batch_adjoint_memory_kept_resized_and_released(vt, gpu):
+    # The last group's recording stays allocated for the next call: calls
+    # that need more of it (a longer segment), less, or none kept at all (after
+    # release_adjoint) give the gradients of a fresh batch.
+    meshes, b = _build(vt, CFG)
+    U0 = [b.state(i) for i in range(b.n)]
+    tg = _targets(b, DT * NSTEPS)
+
+    def run(bb, every, nsteps=NSTEPS):
+        for i in range(bb.n):
+            bb.set_state(i, U0[i])
+        bb.time = 0.0
+        return bb.adjoint(DT, nsteps, tg, every=every)
+
+    def same(x, y):
+        return np.array_equal(x[0], y[0]) and all(np.array_equal(a, c) for a, c in zip(x[1], y[1]))
+
+    base = run(b, EVERY)
+    assert same(run(b, 16), base)      # longer segments than the kept recording holds
+    assert same(run(b, 4), base)       # more checkpoints, shorter segments
+    b.release_adjoint()
+    assert same(run(b, EVERY), base)
+    b.release_adjoint()
+    b.release_adjoint()                # releasing nothing is allowed
+    _, fresh = _build(vt, CFG)
+    assert same(run(fresh, 16), base)
+    fresh.close()
+    b.close()
+
+
+def test_single_mesh_release_adjoint(vt, gpu):
+    m = vt.generate_mesh(nx=6, ny=6)
+    s = vt.Solver(m, 3, 8)
+    s.initialize_hydrodinamics(0)
+    U0 = s.state()
+    lamT = np.random.default_rng(3).standard_normal(U0.shape)
+    s.adjoint_forward(DT, NSTEPS, EVERY)
+    assert s.adjoint_recording() != 0
+    lam0 = s.adjoint_backward(lamT)
+    s.release_adjoint()
+    assert s.adjoint_recording() == 0
+    with pytest.raises(vt.VortexError):
+        s.adjoint_backward(lamT)
+    s.set_state(U0)
+    s.adjoint_forward(DT, NSTEPS, EVERY)
+    assert np.array_equal(s.adjoint_backward(lamT), lam0)
+    s.close()
+
+
 def test_batch_parameter_gradient(vt, gpu):
     meshes, b = _build(vt, CFG)
     eps, x0, y0 = ([c[k] for c in CFG] for k in (3, 4, 5))

@@ -111,6 +111,8 @@ def lib():
             "vt_adjoint_backward": (ctypes.c_int, [vp, _dp, _dp]),
             "vt_adjoint_recording": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_ulonglong)]),
             "vt_set_adjoint_overlap": (ctypes.c_int, [vp, ctypes.c_int]),
+            "vt_adjoint_release": (ctypes.c_int, [vp]),
+            "vt_batch_adjoint_release": (ctypes.c_int, [vp]),
             "vt_vortex_tangent": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
             "vt_mass_l2_objective": (ctypes.c_int, [vp, _dp, _dp, _dp]),
             "vt_adjoint": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
@@ -452,6 +454,12 @@ class Solver:
         stream while sweeping the current one (default on; the same results)."""
         _check(lib().vt_set_adjoint_overlap(self._h, int(bool(on))))
 
+    def release_adjoint(self):
+        """Frees the adjoint memory (checkpoints, segment stage states) the
+        solver otherwise keeps for re-use until close(); the recording is
+        gone, so adjoint_backward() needs a new adjoint_forward()."""
+        _check(lib().vt_adjoint_release(self._h))
+
     def adjoint_backward(self, lamT):
         """(dU_T/dU_0)^T lamT over the last recorded forward run."""
         lamT = np.ascontiguousarray(lamT, dtype=np.float64).reshape(self.ne, self.np, 4)
@@ -705,6 +713,11 @@ class Batch:
         _check(lib().vt_batch_adjoint_vortex(self._h, dt, nsteps, every, None if rt is None else _d(rt), _d(e), _d(a),
                                              _d(c), _d(J), _d(grad)), self)
         return J, grad
+
+    def release_adjoint(self):
+        """Frees the adjoint memory adjoint() and adjoint_vortex() keep for
+        re-use by the next call (the last group's recording) until close()."""
+        _check(lib().vt_batch_adjoint_release(self._h))
 
     def vortex_gradient(self, t, epsilon, x0, y0, dJdU0):
         """dJ_i/d(epsilon_i, x0_i, y0_i) [n][3] from the per-instance dJ/dU0

@@ -400,6 +400,12 @@ int vt_adjoint_forward(vt_ctx* c, double dt, int nsteps, int every) {
     return VT_OK;
 }
 
+int vt_adjoint_release(vt_ctx* c) {
+    if (!c) return VT_ERR_ARG;
+    vti::free_adjoint(c);
+    return VT_OK;
+}
+
 int vt_set_adjoint_overlap(vt_ctx* c, int on) {
     if (!c) return VT_ERR_ARG;
     c->adj_overlap = on != 0;

@@ -326,13 +326,20 @@ int group_vortex_gradient(vt_batch* b, int gi, double t, const double* eps, cons
 
 // The ensemble adjoint, group by group: each group is one mesh (the disjoint
 // union of its instances), recorded, evaluated and swept by the single-mesh
-// adjoint; gather(gi) then reads the group's dJ/dU0 from its lam planes before
-// the group's recording (the bulk of device memory here) is released.
+// adjoint; gather(gi) then reads the group's dJ/dU0 from its lam planes.  A
+// group's recording (checkpoints, segment states, adjoint planes: the bulk of
+// device memory here) is released before another group records, so one is
+// held at a time; the last one is kept and re-used by the next call (freeing
+// and re-allocating some hundred GB costs seconds), until
+// vt_batch_adjoint_release or vt_batch_destroy.
 template <class Gather>
 int batch_adjoint(vt_batch* b, double dt, int nsteps, int every, const double* rho_target, double* J, Gather gather) {
     const std::vector<size_t> off = node_offsets(b);
-    for (int gi = 0; gi < static_cast<int>(b->g.size()); ++gi) {
+    const int ng = static_cast<int>(b->g.size());
+    for (int gi = 0; gi < ng; ++gi) {
         vt_ctx* c = b->g[gi];
+        for (int gj = 0; gj < ng; ++gj)
+            if (gj != gi) vti::free_adjoint(b->g[gj]);
         int rc = vt_adjoint_forward(c, dt, nsteps, every);
         if (rc == VT_ERR_NONPHYSICAL) {
             const int le = c->last_err[0];
@@ -346,8 +353,10 @@ int batch_adjoint(vt_batch* b, double dt, int nsteps, int every, const double* r
         if (!rc) rc = group_objective(b, gi, rho_target, off, J);
         if (!rc) rc = vti::backward_sweep(c);
         if (!rc) rc = gather(gi, off);
-        vti::free_adjoint(c);
-        if (rc) return rc;
+        if (rc) {
+            vti::free_adjoint(c);
+            return rc;
+        }
     }
     for (int s = 0; s < nsteps; ++s) b->time += dt;
     return VT_OK;
@@ -676,6 +685,12 @@ int vt_batch_mass_l2_objective(vt_batch* b, const double* rho_target, double* J,
         if (!rc && dJdU) rc = scatter_group_gradient(b, gi, off, dJdU);
         if (rc) return rc;
     }
+    return VT_OK;
+}
+
+int vt_batch_adjoint_release(vt_batch* b) {
+    if (!b) return VT_ERR_ARG;
+    for (vt_ctx* c : b->g) vti::free_adjoint(c);
     return VT_OK;
 }
 
