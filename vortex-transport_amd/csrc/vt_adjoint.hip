@@ -139,6 +139,7 @@ hipError_t recompute_step(vt_ctx* c, double dt, double* S0, double* S1, double* 
         prm.Uin = in[s];
         prm.Uout = out[s];
         prm.Un = (s == 1 || s == 2) ? S0 : nullptr;
+        prm.U2 = s == 2 ? S1 : nullptr;
         prm.mode = vt::kModeRk1 + s;
         prm.stage_id = stage_base + s;
         hipError_t e = vt::launch_stage(prm, c->p, c->nq, c->cfg, st);

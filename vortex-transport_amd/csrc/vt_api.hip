@@ -138,6 +138,7 @@ hipError_t enqueue_rk4_step(vt_ctx* c, double dt, int stage_base) {
         prm.Uin = st[s].in;
         prm.Uout = st[s].out;
         prm.Un = (s == 1 || s == 2) ? c->Un : nullptr;
+        prm.U2 = s == 2 ? c->Ua : nullptr;
         prm.acc = c->acc;
         prm.mode = st[s].mode;
         prm.stage_id = stage_base + s;
@@ -672,6 +673,7 @@ int vt_stage_kernel_times(vt_ctx* c, double dt, int nsteps, double* ms) {
             prm.Uin = in[s];
             prm.Uout = outp[s];
             prm.Un = (s == 1 || s == 2) ? c->Un : nullptr;
+            prm.U2 = s == 2 ? c->Ua : nullptr;
             prm.acc = c->acc;
             prm.mode = vt::kModeRk1 + s;
             prm.stage_id = 4 * i + s;

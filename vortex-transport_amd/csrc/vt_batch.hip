@@ -57,7 +57,7 @@ void drop_graphs(vt_batch* b) {
 
 // One batched launch per quadrature size for one stage (or, in launch mode 2,
 // one launch per group on concurrent streams).  bufs(c) gives the stage's
-// {Uin, Uout, Un} of group context c.
+// {Uin, Uout, Un, U2} of group context c.
 template <class Bufs>
 hipError_t enqueue_stage(vt_batch* b, double dt, int mode, int stage_id, Bufs bufs) {
     if (b->cfg.persistent == 2) {
@@ -66,12 +66,13 @@ hipError_t enqueue_stage(vt_batch* b, double dt, int mode, int stage_id, Bufs bu
         one.block = b->cfg.block;
         for (size_t gi = 0; gi < b->g.size() && e == hipSuccess; ++gi) {
             vt_ctx* c = b->g[gi];
-            const double* io[3];
+            const double* io[4];
             bufs(c, io);
             vt::StageParams prm{};
             prm.Uin = io[0];
             prm.Uout = const_cast<double*>(io[1]);
             prm.Un = io[2];
+            prm.U2 = io[3];
             prm.acc = c->acc;
             prm.vtx = c->vtx;
             prm.nbr = c->nbr;
@@ -100,11 +101,12 @@ hipError_t enqueue_stage(vt_batch* b, double dt, int mode, int stage_id, Bufs bu
         for (int i = 0; i < bp.ngroups; ++i) {
             vt_ctx* c = b->g[L[i]];
             vt::StageParams& prm = bp.g[i];
-            const double* io[3];
+            const double* io[4];
             bufs(c, io);
             prm.Uin = io[0];
             prm.Uout = const_cast<double*>(io[1]);
             prm.Un = io[2];
+            prm.U2 = io[3];
             prm.acc = c->acc;
             prm.vtx = c->vtx;
             prm.nbr = c->nbr;
@@ -138,6 +140,7 @@ hipError_t enqueue_rk4_step(vt_batch* b, double dt, int stage_base) {
             io[0] = in[s];
             io[1] = out[s];
             io[2] = (s == 1 || s == 2) ? c->Un : nullptr;
+            io[3] = s == 2 ? c->Ua : nullptr;
         });
         if (e != hipSuccess) return e;
     }
@@ -149,6 +152,7 @@ hipError_t enqueue_euler_step(vt_batch* b, double dt, bool from_a, int stage_id)
         io[0] = from_a ? c->Ua : c->Un;
         io[1] = from_a ? c->Un : c->Ua;
         io[2] = nullptr;
+        io[3] = nullptr;
     });
 }
 

@@ -12,18 +12,19 @@ constexpr int kMaxGroups = 4;
 
 enum StageMode : int {
     kModeRhs = 0,    // Uout = dU/dt                         (evolve_elem)
-    kModeRk1 = 1,    // acc = k1, Uout = Un + k1/2                       (Timestepping.cpp:136-171)
-    kModeRk2 = 2,    // Uout = Un + k2/2, Un = Uin - acc/2 (acc kept)    (:178-213)
-    kModeRk3 = 3,    // acc = Un + (k1 + 2 k2 + 2 k3)/6, k2 = 2 (Uin - Un),
-                     // Uout = Un + k3                                   (:220-255)
+    kModeRk1 = 1,    // Uout = Un + k1/2 (Uin = Un)                      (Timestepping.cpp:136-171)
+    kModeRk2 = 2,    // Uout = Un + k2/2                                 (:178-213)
+    kModeRk3 = 3,    // acc = Un + (k1 + 2 k2 + 2 k3)/6 with k1 = 2 (U2 - Un),
+                     // k2 = 2 (Uin - Un); Uout = Un + k3                (:220-255)
     kModeRk4 = 4,    // Uout = acc + k4/6                                (:262-315)
     kModeEuler = 5,  // Uout = Uin + dt dU/dt                (Timestepping.cpp:50-93)
 };
 
 struct StageParams {
     const double* Uin;    // stage input, planes [4*NP][ld]
-    const double* Un;     // step start state (mode 3)
-    double* acc;          // RK accumulator (modes 1-4)
+    const double* Un;     // step start state (modes 2, 3)
+    const double* U2;     // the step's second stage state, Un + k1/2 (mode 3; may be Uout)
+    double* acc;          // RK accumulator (modes 3, 4)
     double* Uout;         // stage output
     const double* vtx;    // vertices, planes [6][ld]: x0 y0 x1 y1 x2 y2
     const int* nbr;       // neighbour across side 1, 2, 3: planes [3][ld]

@@ -446,26 +446,22 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
     if (mode == kModeRk2 || mode == kModeRk3) {
         // Two halves of the variables: the second half's loads are issued once
         // the first half's registers are free.
-        //   stage 2: U3 = Un + k2/2 with Un = U2 - k1/2 recovered from the staged
-        //            U2 and acc = k1; neither Un is read nor acc written.  With
-        //            |k1/2| << |Un| the recovery returns Un itself unless the
-        //            sum U2 rounded to a tie or into the next binade, where it
-        //            can be one ulp off;
+        //   stage 2: U3 = Un + k2/2;
         //   stage 3: acc = Un + (k1 + 2 k2 + 2 k3)/6 and U4 = Un + k3, with
-        //            k2 = 2 (U3 - Un) recovered from the staged U3 (the
-        //            difference is exact when U3 and Un are within a factor of
-        //            two, Sterbenz; k2 then carries U3's rounding).
+        //            k1 = 2 (U2 - Un) and k2 = 2 (U3 - Un) recovered from the
+        //            stored U2 and the staged U3, so that no stage stores k1
+        //            (the differences are exact when the two states are within
+        //            a factor of two, Sterbenz; k1 and k2 then carry the
+        //            rounding of U2 and U3).  U2 may be read from Uout: each
+        //            thread reads its own element's values before writing them.
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            double ac[2 * NP], u0[2 * NP];
+            double ua[2 * NP], u0[2 * NP];
 #pragma unroll
-            for (int t = 0; t < 2 * NP; ++t) ac[t] = __builtin_nontemporal_load(&uplane_at(prm.acc, 2 * h * NP + t, ld8, oe));
-            if (mode == kModeRk2) {
+            for (int t = 0; t < 2 * NP; ++t) u0[t] = __builtin_nontemporal_load(&uplane_at(prm.Un, 2 * h * NP + t, ld8, oe));
+            if (mode == kModeRk3) {
 #pragma unroll
-                for (int t = 0; t < 2 * NP; ++t) u0[t] = my[(2 * h * NP + t) * BS] - ac[t] * 0.5;
-            } else {
-#pragma unroll
-                for (int t = 0; t < 2 * NP; ++t) u0[t] = __builtin_nontemporal_load(&uplane_at(prm.Un, 2 * h * NP + t, ld8, oe));
+                for (int t = 0; t < 2 * NP; ++t) ua[t] = __builtin_nontemporal_load(&uplane_at(prm.U2, 2 * h * NP + t, ld8, oe));
             }
             // Node-major, so each row of Minv is read once per half and dies; the
             // table base is made opaque per half so that its loads are not
@@ -481,8 +477,9 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                     if (mode == kModeRk2) {
                         __builtin_nontemporal_store(u0[t] + kk * 0.5, &uplane_at(prm.Uout, v, ld8, oe));
                     } else {
+                        const double k1 = 2.0 * (ua[t] - u0[t]);
                         const double k2 = 2.0 * (my[v * BS] - u0[t]);
-                        __builtin_nontemporal_store(u0[t] + c6 * ((ac[t] + 2.0 * k2) + 2.0 * kk), &uplane_at(prm.acc, v, ld8, oe));
+                        __builtin_nontemporal_store(u0[t] + c6 * ((k1 + 2.0 * k2) + 2.0 * kk), &uplane_at(prm.acc, v, ld8, oe));
                         __builtin_nontemporal_store(u0[t] + kk, &uplane_at(prm.Uout, v, ld8, oe));
                     }
                 }
@@ -499,14 +496,13 @@ __device__ __forceinline__ void stage_tile(const StageParams& prm, int tile, dou
                 const int v = k * NP + i;
                 __builtin_nontemporal_store(ac[v] + c6 * (dt * dudt(i, k)), &uplane_at(prm.Uout, v, ld8, oe));
             }
-    } else if (mode == kModeRk1) {  // k1 = dt f(Un); acc = k1; U2 = Un + k1/2
+    } else if (mode == kModeRk1) {  // k1 = dt f(Un); U2 = Un + k1/2
 #pragma unroll
         for (int i = 0; i < NP; ++i)
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int v = k * NP + i;
                 const double kk = dt * dudt(i, k);
-                __builtin_nontemporal_store(kk, &uplane_at(prm.acc, v, ld8, oe));
                 __builtin_nontemporal_store(my[v * BS] + kk * 0.5, &uplane_at(prm.Uout, v, ld8, oe));
             }
     } else if (mode == kModeEuler) {  // U + dt dU/dt
