@@ -16,8 +16,9 @@ ensemble, one launch per RK stage, against per-instance plain solvers),
 "adjoint" (forward + checkpointed discrete adjoint on 256 x 256 cells, against
 the forward solve alone), "batched_adjoint" (32 uneven instances up to
 512 x 512 cells, recorded forward + reverse sweep + per-instance parameter
-gradients, against the batched forward) and "large_mesh" (a forward-only
-1024 x 1024-cell run), "c1_full_run" and "shipped_input_run" (the
+gradients, against the batched forward), "large_mesh" (a forward-only
+1024 x 1024-cell run), "full_run" (the main workload's 2,000 configured
+steps to the end), "c1_full_run" and "shipped_input_run" (the
 reference's own 8 x 8 case to the end at integration orders 8 and 6, the
 latter its shipped input, beside the rk4 times the reference recorded for it).
 
@@ -79,6 +80,7 @@ def parse_args():
     ap.add_argument("--batched-adjoint-steps", type=int, default=500)
     ap.add_argument("--batched-adjoint-every", type=int, default=16, help="checkpoint interval of the ensemble adjoint")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-full-run", action="store_true", help="skip the 2,000-step run of the main workload")
     ap.add_argument("--no-shard", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
@@ -512,6 +514,42 @@ def large_mesh_figure(vt, steps, dt=5e-5, n=1024):
                         "initial_state_device": c4 - c3}}
 
 
+def full_run_figure(vt, steps=2000, dt=1e-4, n=512):
+    """The main figure's workload run to its configured 2,000 steps (dt 1e-4,
+    T = 0.2) on the resident state: time to solution, the run's DOF-stage
+    rate, the L2 state error against the exactly advected vortex at T and the
+    relative drift of the total mass sum_e detJ_e w^T rho_e (w the column
+    sums of M_ref) over the run."""
+    import torch
+    mesh = vt.generate_mesh_seeded(n, n, 20.0, 20.0, 0.02, 2)
+    s = vt.Solver(mesh, 3, 8)
+    s.initialize_hydrodinamics(0)
+    s.rk4(dt, 4)
+    s.prepare(1, dt, steps)
+    settle_clocks(s, dt, 0.2)
+    s.time = 0.0
+    s.initialize_hydrodinamics(0)
+    v = mesh.element_coordinates
+    detJ = (v[:, 1, 0] - v[:, 0, 0]) * (v[:, 2, 1] - v[:, 0, 1]) - (v[:, 1, 1] - v[:, 0, 1]) * (v[:, 2, 0] - v[:, 0, 0])
+    w = vt.reference_operators(3, 8)[0].sum(axis=0)
+    mass = lambda U: float(np.sum(detJ[:, None] * (U[:, :, 0] * w[None, :])))
+    m0 = mass(s.state())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.rk4(dt, steps)
+    wall = time.perf_counter() - t0
+    U = s.state()
+    err = s.l2_state_error(s.exact_vortex(s.time))
+    ne = s.ne
+    s.close()
+    return {"workload": "%dx%d cells (%d triangles), 20x20 perturbed mesh (seed 2), p=3, integration_order 8, "
+                        "%d RK4 steps of dt %g" % (n, n, ne, steps, dt),
+            "steps": steps, "seconds": wall, "ms_per_step": wall / steps * 1e3,
+            "value": ne * 10 * 4 * 4 * steps / wall, "unit": "DOF-stage updates/s",
+            "l2_state_error_vs_exact": [float(x) for x in err],
+            "mass_drift_rel": abs(mass(U) - m0) / abs(m0), "finite": bool(np.isfinite(U).all())}
+
+
 def pcie_figure(vt, steps, dt=1e-4, n=512):
     """The workload stepped as the drop-in's per-step rk4() does it: the host
     state goes to the device, one RK4 step runs, and U and the nodal flux F come
@@ -775,6 +813,8 @@ def main():
         line["batched_adjoint"] = batched_adjoint_figure(vt, a.batched_adjoint_steps, every=a.batched_adjoint_every)
     if not a.no_large:
         line["large_mesh"] = large_mesh_figure(vt, a.large_steps)
+    if not a.no_full_run:
+        line["full_run"] = full_run_figure(vt)
     if not a.no_shard:
         line["shard"] = shard_figure(vt, 200)
     if not a.no_pcie:
